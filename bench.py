@@ -1,0 +1,144 @@
+"""Benchmark: G ray-samples/s of the gfx950 DRR caster on one MI355X.
+
+Workload (N=1): 256^3 int16 CT phantom, cone beam SID 1000 / SOD 600 mm,
+1024x1024 float32 exp(-L) detector, step 0.5 voxel (config c2).  One step =
+one full-detector render with the volume already resident in HBM.
+
+Prints ONE JSON line.  `value` = N_samples / (seconds per render) / 1e9, where
+N_samples is the plain-march ray-sample count (sum over pixels of
+floor(num_steps - ray offset)).  `roofline` prices the caster kernel's
+gather-equivalent bytes (8 voxels x 2 B per sample + 4 B per pixel) against
+the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (FP32, all granted
+host threads) on the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def bench_drr(scene, steps: int, warmup: int):
+    import torch
+
+    from robartsvtk_amd import abi, configs
+
+    ctx = abi.Context(0)
+    vol = abi.Volume.from_array(ctx, scene.volume, scene.spacing)
+    view = scene.views[0]
+    mp = configs.march_params(scene.march)
+    dp = configs.drr_params(scene.drr)
+    out = torch.empty((view.height, view.width), dtype=torch.float32, device="cuda")
+    n_samples = abi.count_ray_steps(ctx, vol, view, mp)
+    for _ in range(warmup):
+        abi.render_drr(ctx, vol, view, mp, dp, out)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for a, b in ev:
+        a.record(stream)
+        abi.render_drr(ctx, vol, view, mp, dp, out)
+        b.record(stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t0.elapsed_time(t1)
+    kern_ms = statistics.fmean(a.elapsed_time(b) for a, b in ev)
+    img = out.cpu().numpy()
+    vol.close()
+    ctx.close()
+    return {"n_samples": n_samples, "ms_per_step": total_ms / steps, "kernel_ms": kern_ms, "pixels": view.width * view.height,
+            "image_mean": float(img.mean())}
+
+
+def cpu_baseline(scene, n_samples: int, runs: int = 3):
+    from oracle import oracle as orc
+    from robartsvtk_amd import configs
+
+    th = cpu_threads()
+    vd = configs.view_dict(scene.views[0])
+    orc.drr(scene.volume, scene.spacing, vd, scene.march, scene.drr, "f32", nthreads=th)  # warm-up
+    ts = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        orc.drr(scene.volume, scene.spacing, vd, scene.march, scene.drr, "f32", nthreads=th)
+        ts.append(time.perf_counter() - t)
+    med = statistics.median(ts)
+    return {"value": round(n_samples / med / 1e9, 4), "unit": "G ray-samples/s", "cores": th, "kind": "port",
+            "sample": f"full c2 frame (1024x1024 rays, {n_samples} samples), FP32 oracle, median of {runs}, "
+                      f"{med:.3f} s each"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build runs every kernel on a single MI355X and has no "
+              "multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    from robartsvtk_amd import configs
+
+    log("building c2 scene")
+    scene = configs.scene_c2(scale=1.0)
+    log("timing DRR caster")
+    r = bench_drr(scene, args.steps, args.warmup)
+    sec = r["ms_per_step"] / 1e3
+    value = r["n_samples"] / sec / 1e9
+    alg_bytes = r["n_samples"] * 8 * 2 + r["pixels"] * 4
+    achieved = alg_bytes / (r["kernel_ms"] / 1e3) / 1e9
+    line = {
+        "metric": "G ray-samples/s (DRR caster, c2)",
+        "value": round(value, 3),
+        "unit": "G ray-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c2: DRR of a 256^3 int16 CT phantom (seeded), cone beam SID 1000 / SOD 600 mm, "
+                               "1024x1024 float32 exp(-L), step 0.5 voxel, skip rule on, precise march",
+                   "n_samples": r["n_samples"], "voxel_bytes": 2},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "drr_kernel<short,true,true>", "kernel_ms": round(r["kernel_ms"], 4),
+                     "alg_bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu_baseline:
+        log("timing CPU oracle baseline")
+        line["cpu_baseline"] = cpu_baseline(scene, r["n_samples"])
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

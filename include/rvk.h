@@ -1,0 +1,219 @@
+/*
+ * rvk.h -- C-ABI of the MI355X (gfx950) ray-casting library librvkhip.so.
+ *
+ * It replaces the nvcc-compiled "renderAlgo" boundary of RobartsVTK's CUDA
+ * volume mappers, and the CUDA device/stream handling under it.  Paths below
+ * are relative to the reference root; V/ = Visualization/vtkCudaVisualization,
+ * C/ = Common/vtkCudaCommon.
+ *
+ * Conventions (differences from the reference are deliberate):
+ *   - every entry point returns an rvk_status (0 = OK) and never throws; the
+ *     reference returned `cudaGetLastError()==0` as a bool
+ *     (e.g. V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:302);
+ *   - there is no process-global device state: per-render parameters are
+ *     kernel arguments, not __constant__ symbols and texture references
+ *     (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:12-19), so calls on different
+ *     contexts need no mutex;
+ *   - the library owns device allocations behind opaque handles; render
+ *     outputs are caller-owned device pointers written asynchronously on the
+ *     context's stream;
+ *   - volumes keep their native scalar type on the device (no float widening).
+ */
+#ifndef RVK_H
+#define RVK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RVK_VERSION 100
+
+/* At most this many frames per volume (V/vtkCudaVolumeMapper.h:59). */
+#define RVK_MAX_FRAMES 30
+
+typedef enum rvk_status {
+  RVK_OK = 0,
+  RVK_ERR_INVALID_ARGUMENT = 1,
+  RVK_ERR_NO_DEVICE = 2,
+  RVK_ERR_HIP = 3,
+  RVK_ERR_OUT_OF_MEMORY = 4,
+  RVK_ERR_NOT_READY = 5,
+  RVK_ERR_UNSUPPORTED = 6
+} rvk_status;
+
+/* Voxel scalar types accepted by SetInputData (V/vtkCuda1DVolumeMapper.cxx:107-187). */
+typedef enum rvk_dtype {
+  RVK_U8 = 0, RVK_I8 = 1, RVK_U16 = 2, RVK_I16 = 3, RVK_U32 = 4, RVK_I32 = 5, RVK_F32 = 6
+} rvk_dtype;
+
+/* DRR output modes.  RGBA8_REF is the reference's tinted image whose alpha
+ * carries 256*(1 - offset*exp(-L)) (V/CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo.cuh:114-115,163-166). */
+typedef enum rvk_drr_output {
+  RVK_DRR_RGBA8_REF = 0,
+  RVK_DRR_F32_EXP_NEG_L = 1, /* float exp(-L) */
+  RVK_DRR_F32_L = 2          /* float L, the line integral */
+} rvk_drr_output;
+
+typedef struct rvk_ctx rvk_ctx;
+typedef struct rvk_volume rvk_volume;
+typedef struct rvk_tf1d rvk_tf1d;
+typedef struct rvk_tf2d rvk_tf2d;
+
+/* One view.  view_to_voxels is the pre-baked screen->voxel matrix of
+ * vtkCudaRendererInformationHandler::SetViewToVoxelsMatrix
+ * (V/vtkCudaRendererInformationHandler.cxx:112-139), row major: pixel (x,y)
+ * maps through (vx, vy, z, 1) with vx = flipped ? x/W : 1-x/W, vy = y/H.
+ * Planes are (a,b,c,d) in voxel space as FigurePlanes makes them
+ * (V/vtkCudaRendererInformationHandler.cxx:216-260). */
+typedef struct rvk_view {
+  float view_to_voxels[16];
+  int32_t width, height;
+  int32_t flipped;
+  int32_t n_clip_planes;     /* 0..6 */
+  float clip_planes[24];
+  int32_t n_keyhole_planes;  /* 0..6 */
+  float keyhole_planes[24];
+  const float* zbuffer;      /* device pointer or NULL (= depth 1, no geometry) */
+  int32_t zb_width, zb_height;
+} rvk_view;
+
+/* March parameters.  Reference defaults: step_scale 1 (one MinSpacing per
+ * step, V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:414-419), term_T 1/64 (1D,
+ * V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:135) or 1/32 (2D,
+ * V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:313), jitter on, skip rule on. */
+typedef struct rvk_march_params {
+  float step_scale;  /* sample distance in units of the smallest voxel spacing */
+  float term_T;      /* stop a ray once its transmittance drops below term_T */
+  int32_t jitter;    /* 1: per-pixel 16x16 ray-offset table, 0: none */
+  int32_t skip_rule; /* 1: reference two-step skip/back-step, 0: plain march */
+  int32_t precise;   /* 0: reference FP32 march (p += inc each step); 1: sample
+                        positions from the step index and a compensated DRR
+                        line integral (no drift over long rays) */
+} rvk_march_params;
+
+/* DRR parameters: mu = slope*value + intercept per mm; reference defaults
+ * 0.022 / 0.000022 / 1.0 (V/vtkCudaDRRImageVolumeMapper.cxx:21-23). */
+typedef struct rvk_drr_params {
+  float ct_intercept, ct_slope, ct_offset;
+  int32_t output;          /* rvk_drr_output */
+  int32_t ref_termination; /* 1: stop once L >= ln(256*ct_offset) */
+  uint8_t tint[4];         /* RGBA tint for RVK_DRR_RGBA8_REF */
+} rvk_drr_params;
+
+/* 1D TF scalars: the POD half of cuda1DTransferFunctionInformation
+ * (V/CUDA_container1DTransferFunctionInformation.h:20-39). */
+typedef struct rvk_tf1d_params {
+  float intensity_low, intensity_multiplier;
+  float gradient_low, gradient_multiplier; /* non-finite multiplier: no gradient opacity */
+  float ambient, diffuse, specular, specular_power;
+} rvk_tf1d_params;
+
+/* 2D TF scalars (V/CUDA_container2DTransferFunctionInformation.h:20-27). */
+typedef struct rvk_tf2d_params {
+  float intensity_low, intensity_multiplier;
+  float gradient_low, gradient_multiplier, gradient_offset;
+} rvk_tf2d_params;
+
+/* Optional per-pixel side outputs of the compositing casters (device
+ * pointers, each W*H floats, any may be NULL): steps taken, and the min/max
+ * ray depths formRays writes (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:373,387). */
+typedef struct rvk_aux_out {
+  float* depth;
+  float* min_depth;
+  float* max_depth;
+} rvk_aux_out;
+
+/* Cel and distance shading constants as the renderer handler stores them:
+ * darkness, start, 1/(stop-start) (V/vtkCudaRendererInformationHandler.cxx:79-99). */
+typedef struct rvk_shade_params {
+  float celr, cela, celc;
+  float disr, disa, disc;
+} rvk_shade_params;
+
+/* ---- errors -------------------------------------------------------------- */
+/* Message of the last failing call on this thread ("" if none). */
+const char* rvk_last_error_string(void);
+const char* rvk_status_string(int status);
+int rvk_version(void);
+
+/* ---- device / stream context ----------------------------------------------
+ * Replaces CudaObject + vtkCudaDeviceManager (C/CudaObject.h:44-51,
+ * C/vtkCudaDeviceManager.cxx:249-296,365-381): one context = one device and
+ * one stream.  rvk_ctx_create makes its own stream (default priority);
+ * rvk_ctx_create_on_stream borrows a caller's hipStream_t (NULL = the
+ * device's default stream). */
+int rvk_device_count(int* count);
+int rvk_ctx_create(int device, rvk_ctx** out);
+int rvk_ctx_create_on_stream(int device, void* hip_stream, rvk_ctx** out);
+int rvk_ctx_destroy(rvk_ctx* ctx);
+int rvk_ctx_device(const rvk_ctx* ctx, int* device);
+int rvk_ctx_stream(const rvk_ctx* ctx, void** hip_stream);
+int rvk_ctx_set_stream(rvk_ctx* ctx, void* hip_stream); /* NULL: back to the context's own stream */
+int rvk_ctx_sync(rvk_ctx* ctx);                          /* CallSyncThreads */
+
+/* ---- volumes --------------------------------------------------------------
+ * Replaces CUDA_castBuffer<T,float> + *_loadImageInfo / changeFrame /
+ * clearImageArray (V/CUDA_vtkCudaVolumeMapper_renderAlgo.h:30-35,
+ * V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.h:74-77).  Frames keep their native
+ * type in a 4x4x4-bricked device layout; src is x-fastest (VTK order) host
+ * or device memory, borrowed only for the duration of the call. */
+int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[3], int dtype,
+                      int n_frames, rvk_volume** out);
+int rvk_volume_set_frame(rvk_volume* vol, int frame, const void* src, int src_on_device);
+/* u8 gradient-magnitude companion volume, |grad| = value*scale, computed on
+ * the device by central differences (0.5/spacing), scale = max_gradient/255. */
+int rvk_volume_compute_gradmag(rvk_volume* vol, int frame, float max_gradient);
+int rvk_volume_destroy(rvk_volume* vol);
+int rvk_volume_dims(const rvk_volume* vol, int32_t dims[3], double spacing[3], int* dtype, int* n_frames);
+
+/* ---- transfer functions -----------------------------------------------------
+ * 1D: replaces CUDA_vtkCuda1DVolumeMapper_renderAlgo_loadTextures
+ * (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.h:61-64).  Tables of n floats
+ * (n <= 4096); galpha may be NULL (all ones). */
+int rvk_tf1d_create(rvk_ctx* ctx, int n, const float* alpha, const float* galpha, const float* red,
+                    const float* green, const float* blue, const rvk_tf1d_params* params, rvk_tf1d** out);
+int rvk_tf1d_destroy(rvk_tf1d* tf);
+/* 2D: replaces CUDA_vtkCuda2DVolumeMapper_renderAlgo_loadTextures
+ * (V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.h:30-80).  tables[8] are n*n
+ * floats indexed g*n+i, in the order alpha, red, green, blue, ambient,
+ * diffuse, specular, specular power.  keyhole_tables may be NULL. */
+int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const float* const keyhole_tables[8],
+                    const rvk_tf2d_params* params, rvk_tf2d** out);
+int rvk_tf2d_destroy(rvk_tf2d* tf);
+
+/* ---- renders ----------------------------------------------------------------
+ * Each fuses formRays into the caster (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:391-447).
+ * rvk_render_1d replaces CUDA_vtkCuda1DVolumeMapper_renderAlgo_doRender
+ * (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.h:29-33); out is W*H RGBA8. */
+int rvk_render_1d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf1d* tf, const rvk_view* view,
+                  const rvk_march_params* march, uint8_t* out_rgba, const rvk_aux_out* aux);
+/* Replaces CUDA_vtkCuda2DVolumeMapper_renderAlgo_doRender
+ * (V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.h:30-34).  use_gradmag = 1 reads
+ * the gradient magnitude from rvk_volume_compute_gradmag's volume (shading
+ * then takes the Phong-Lambert factor as 1). */
+int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d* tf, const rvk_view* view,
+                  const rvk_march_params* march, int use_gradmag, uint8_t* out_rgba, const rvk_aux_out* aux);
+/* Replaces CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo_doRender
+ * (V/CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo.h:30-35).  out is W*H
+ * RGBA8 or W*H floats according to drr->output. */
+int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_view* view,
+                   const rvk_march_params* march, const rvk_drr_params* drr, void* out);
+/* normBuffer + doCelShade (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:449-512)
+ * in place on an RGBA8 image; depth/min/max from rvk_aux_out.  Neighbours
+ * past the last row/column are clamped (the reference reads past them). */
+int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
+                   const float* min_depth, const float* max_depth, uint8_t* rgba);
+
+/* Ray-sample count of a plain march without opacity termination: sum over
+ * pixels of floor(num_steps - ray offset) (the DRR's sample count, from the
+ * geometry alone).  Synchronous; *out is a host pointer. */
+int rvk_count_ray_steps(rvk_ctx* ctx, const rvk_volume* vol, const rvk_view* view, const rvk_march_params* march,
+                        int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

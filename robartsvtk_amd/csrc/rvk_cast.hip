@@ -1,0 +1,692 @@
+// rvk_cast.hip -- single-view gfx950 ray casters (DRR, 1D TF, 2D TF).
+//
+// One thread per output pixel; a 256-thread workgroup covers a 16x16 tile and
+// each wave64 an 8x8 quadrant, so a wave's 64 samples at one step fall in a
+// few neighbouring 4x4x4 volume bricks.  formRays is fused into each kernel's
+// prologue (ray state stays in registers instead of 9 SoA arrays), and all
+// parameters are kernel arguments.
+//
+// Reference kernels restated (V/ = Visualization/vtkCudaVisualization):
+//   DRR  V/CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo.cuh:11-171
+//   1D   V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:27-227
+//   2D   V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:30-470
+//   post V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:449-512
+#include <cmath>
+
+#include "rvk_internal.h"
+
+namespace rvk {
+
+// The reference's 16x16 ray-offset table (V/vtkCudaVolumeMapper.cxx:59-93),
+// indexed by (x mod 16) + 16 (y mod 16).  Parity depends on it bit for bit.
+__constant__ float kRayOffsets[256] = {
+  0.70554f, 0.53342f, 0.57951f, 0.28956f, 0.30194f, 0.77474f, 0.01401f, 0.76072f,
+  0.81449f, 0.70903f, 0.04535f, 0.41403f, 0.86261f, 0.79048f, 0.37353f, 0.96195f,
+  0.87144f, 0.05623f, 0.94955f, 0.36401f, 0.52486f, 0.76711f, 0.0535f, 0.59245f,
+  0.4687f, 0.29816f, 0.62269f, 0.64782f, 0.26379f, 0.27934f, 0.8298f, 0.8246f,
+  0.58916f, 0.98609f, 0.91096f, 0.22686f, 0.69511f, 0.98f, 0.24393f, 0.53387f,
+  0.10636f, 0.99941f, 0.67617f, 0.0157f, 0.57518f, 0.10005f, 0.10302f, 0.79888f,
+  0.28448f, 0.04564f, 0.29577f, 0.38201f, 0.30097f, 0.94857f, 0.97982f, 0.40137f,
+  0.27827f, 0.16044f, 0.16282f, 0.64658f, 0.41007f, 0.41276f, 0.71273f, 0.3262f,
+  0.63317f, 0.20756f, 0.18601f, 0.58335f, 0.08071f, 0.45797f, 0.90572f, 0.26136f,
+  0.78521f, 0.3789f, 0.28966f, 0.91937f, 0.63174f, 0.62764f, 0.42845f, 0.09797f,
+  0.56104f, 0.69448f, 0.91371f, 0.83481f, 0.02262f, 0.54336f, 0.91616f, 0.43026f,
+  0.67794f, 0.50245f, 0.51373f, 0.46298f, 0.35347f, 0.40483f, 0.26973f, 0.05559f,
+  0.24384f, 0.97907f, 0.06091f, 0.39029f, 0.36499f, 0.48989f, 0.15566f, 0.47445f,
+  0.25726f, 0.62875f, 0.54207f, 0.1563f, 0.93854f, 0.65449f, 0.50608f, 0.39047f,
+  0.10737f, 0.78399f, 0.45964f, 0.75368f, 0.59609f, 0.83273f, 0.01875f, 0.21036f,
+  0.07395f, 0.10545f, 0.33169f, 0.12824f, 0.00024f, 0.53679f, 0.65705f, 0.54401f,
+  0.82741f, 0.08189f, 0.19192f, 0.67891f, 0.4542f, 0.35702f, 0.14998f, 0.70439f,
+  0.92878f, 0.53021f, 0.08964f, 0.75772f, 0.40184f, 0.46187f, 0.49216f, 0.20762f,
+  0.32973f, 0.09542f, 0.58979f, 0.16987f, 0.92761f, 0.09792f, 0.44386f, 0.27294f,
+  0.87254f, 0.75068f, 0.27294f, 0.67364f, 0.25662f, 0.08989f, 0.03095f, 0.32271f,
+  0.79012f, 0.29725f, 0.23528f, 0.48047f, 0.2546f, 0.3406f, 0.04493f, 0.48242f,
+  0.20601f, 0.86453f, 0.58862f, 0.7549f, 0.92788f, 0.33101f, 0.54294f, 0.08069f,
+  0.63437f, 0.41003f, 0.96042f, 0.11462f, 0.92344f, 0.6202f, 0.34772f, 0.14924f,
+  0.47997f, 0.2194f, 0.99373f, 0.13042f, 0.02888f, 0.34539f, 0.54766f, 0.92295f,
+  0.53824f, 0.40642f, 0.84724f, 0.82622f, 0.67242f, 0.72189f, 0.99677f, 0.3398f,
+  0.49521f, 0.41296f, 0.69528f, 0.17908f, 0.42291f, 0.54317f, 0.81466f, 0.54091f,
+  0.42753f, 0.50906f, 0.22778f, 0.61918f, 0.48983f, 0.68081f, 0.8866f, 0.37051f,
+  0.30249f, 0.29286f, 0.15031f, 0.52982f, 0.22326f, 0.58452f, 0.36345f, 0.87597f,
+  0.47801f, 0.19063f, 0.68406f, 0.74741f, 0.61393f, 0.78213f, 0.16174f, 0.80777f,
+  0.20261f, 0.95676f, 0.06585f, 0.06152f, 0.79319f, 0.3796f, 0.46358f, 0.11954f,
+  0.11547f, 0.17377f, 0.04811f, 0.71481f, 0.53302f, 0.561f, 0.21673f, 0.468f,
+  0.74635f, 0.75231f, 0.39893f, 0.90309f, 0.746f, 0.08855f, 0.63457f, 0.71302f};
+
+__device__ __forceinline__ float ray_offset(int jitter, int x, int y) {
+  return jitter ? kRayOffsets[(x & 15) + 16 * (y & 15)] : 0.0f;
+}
+
+__device__ __forceinline__ float ray_length(const VolGeom& g, const V3<float>& inc) {
+  return sqrtf(inc.x * inc.x * g.sp[0] * g.sp[0] + inc.y * inc.y * g.sp[1] * g.sp[1] +
+               inc.z * inc.z * g.sp[2] * g.sp[2]);
+}
+
+__device__ __forceinline__ void step(V3<float>& p, const V3<float>& inc) {
+  p.x += inc.x; p.y += inc.y; p.z += inc.z;
+}
+
+// Sample position.  Reference mode keeps the incrementally updated p
+// (p += inc per step, as the CUDA kernels do); PRECISE derives it from the
+// step index, p0 + (k_start - k) inc, which removes the drift that repeated
+// FP32 additions build up over long rays.
+template <bool PRECISE>
+__device__ __forceinline__ V3<float> cur_pos(const V3<float>& p, const V3<float>& p0, const V3<float>& inc, int kst,
+                                             int k) {
+  if (!PRECISE) return p;
+  const float t = (float)(kst - k);
+  return V3<float>{p0.x + t * inc.x, p0.y + t * inc.y, p0.z + t * inc.z};
+}
+
+// ===========================================================================
+// DRR
+struct DrrArgs {
+  VolGeom g;
+  ViewGeom v;
+  float step_len;
+  int jitter, skip, ref_term;
+  float intercept, slope, offset;
+  int output;
+  uint32_t tint;
+  const void* vol;
+  void* out;
+  int tiles_x;
+};
+
+// Line integral along one ray (V/...DRR...cuh:11-117).  The accumulator starts
+// at ln(256*offset) in the reference RGBA8 mode and at 0 in the float modes.
+template <typename VT, bool SKIP, bool PRECISE>
+__device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const DrrArgs& a, const Ray& r, int x, int y) {
+  const bool ref_mode = a.output == RVK_DRR_RGBA8_REF;
+  const float lg = logf(256.0f * a.offset);
+  float acc = ref_mode ? lg : 0.0f;
+  const float stop_at = ref_mode ? 0.0f : -lg;
+  const float j = ray_offset(a.jitter, x, y);
+  int k = step_count(r.num_steps - j);
+  V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  const V3<float> inc = r.inc;
+  const float ds = ray_length(a.g, inc);
+  const float ci = a.intercept * -ds;
+  const float cs = a.slope * -ds;
+  const int es = k - (int)ceilf(r.exclude_start);
+  const int ee = k - (int)floorf(r.exclude_end);
+  const int kst = k;
+  const V3<float> p0 = p;
+  float comp = 0.0f;  // Kahan compensation of the line integral (PRECISE)
+  auto add = [&](float v) {
+    if (!PRECISE) { acc += v; return; }
+    const float yk = v - comp;
+    const float tk = acc + yk;
+    comp = (tk - acc) - yk;
+    acc = tk;
+  };
+  int sa = 0, sb = 0;
+  while (k > 0) {
+    const bool in_key = (es >= k && ee < k);
+    const V3<float> q = cur_pos<PRECISE>(p, p0, inc, kst, k);
+    const float alpha = cs * trilinear(vol, a.g, q.x, q.y, q.z) + ci;
+    if (SKIP) {
+      if (alpha < 0.0f && !in_key) {
+        sa = sb;
+        sb = 0;
+        if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
+        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
+        if (!sa) add(alpha);
+        if (a.ref_term && acc <= stop_at) break;
+      } else {
+        if (!sa) { step(p, inc); k--; }
+        sb = !sa;
+        step(p, inc);
+        k--;
+        sa = 0;
+      }
+    } else {
+      step(p, inc);
+      k--;
+      if (alpha < 0.0f && !in_key) {
+        add(alpha);
+        if (a.ref_term && acc <= stop_at) break;
+      }
+    }
+  }
+  if (ref_mode) {
+    const float o = 256.0f - expf(acc);
+    return o < 0.0f ? 0.0f : o;
+  }
+  if (a.output == RVK_DRR_F32_EXP_NEG_L) return expf(acc);
+  return -acc;
+}
+
+template <typename VT, bool SKIP, bool PRECISE>
+__global__ __launch_bounds__(256) void drr_kernel(DrrArgs a) {
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  int x, y;
+  tile_pixel(tile, a.tiles_x, x, y);
+  if (x >= a.v.W || y >= a.v.H) return;
+  const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
+  const float v = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a, r, x, y);
+  const long idx = (long)y * a.v.W + x;
+  if (a.output == RVK_DRR_RGBA8_REF) {
+    // V/...DRR...cuh:163-166
+    const float tw = (float)(a.tint >> 24);
+    const float q = tw / (tw + v);
+    const uint32_t A = to_u8(tw + (1.0f - tw / 255.0f) * v);
+    const uint32_t R = to_u8(q * (float)(a.tint & 255u));
+    const uint32_t G = to_u8(q * (float)((a.tint >> 8) & 255u));
+    const uint32_t B = to_u8(q * (float)((a.tint >> 16) & 255u));
+    ((uint32_t*)a.out)[idx] = R | (G << 8) | (B << 16) | (A << 24);
+  } else {
+    ((float*)a.out)[idx] = v;
+  }
+}
+
+// ===========================================================================
+// 1D transfer function
+struct Cast1dArgs {
+  VolGeom g;
+  ViewGeom v;
+  float step_len, term_T;
+  int jitter;
+  int n;
+  rvk_tf1d_params p;
+  const float* tables;
+  const void* vol;
+  uint32_t* out;
+  float* depth;
+  float* min_depth;
+  float* max_depth;
+  int tiles_x;
+};
+
+// V/...1D...cuh:27-172.  GRAD=false is taken only when the gradient cannot
+// change the result: diffuse = specular = 0, specular power >= 0 and no
+// gradient-opacity table (non-finite multiplier); then shadeD = ambient and
+// shadeS = 0 exactly, whatever the Phong-Lambert factor.
+template <typename VT, bool SKIP, bool GRAD, bool PRECISE>
+__global__ __launch_bounds__(256) void cast1d_kernel(Cast1dArgs a) {
+  extern __shared__ float lds[];
+  const int n = a.n;
+  for (int i = threadIdx.x; i < 5 * n; i += blockDim.x) lds[i] = a.tables[i];
+  __syncthreads();
+  const float* t_alpha = lds;
+  const float* t_galpha = lds + n;
+  const float* t_r = lds + 2 * n;
+  const float* t_g = lds + 3 * n;
+  const float* t_b = lds + 4 * n;
+
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  int x, y;
+  tile_pixel(tile, a.tiles_x, x, y);
+  if (x >= a.v.W || y >= a.v.H) return;
+  const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
+  const VT* __restrict__ vol = (const VT*)a.vol;
+  const VolGeom& g = a.g;
+
+  float Tr = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  const float lo = a.p.intensity_low, mult = a.p.intensity_multiplier;
+  const float glo = a.p.gradient_low, gmult = a.p.gradient_multiplier;
+  const float amb = a.p.ambient, dif = a.p.diffuse, spc = a.p.specular, spw = a.p.specular_power;
+  const bool use_galpha = isfinite(gmult);
+  const float term = a.term_T;
+  const float j = ray_offset(a.jitter, x, y);
+  int k = step_count(r.num_steps - j);
+  const int k0 = k;
+  V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  const V3<float> inc = r.inc;
+  const float rlen = ray_length(g, inc);
+  const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
+  const int es = (int)ceilf(r.exclude_start), ee = (int)floorf(r.exclude_end);
+  const int kst = k;
+  const V3<float> p0 = p;
+  int sa = 0, sb = 0;
+
+  auto accumulate = [&](float alpha, float u) {
+    float shd, shs;
+    if (GRAD) {
+      V3<float> gr;
+      const V3<float> q = cur_pos<PRECISE>(p, p0, inc, kst, k);
+      gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
+      gr.y = (trilinear(vol, g, q.x, q.y + 1.0f, q.z) - trilinear(vol, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
+      gr.z = (trilinear(vol, g, q.x, q.y, q.z + 1.0f) - trilinear(vol, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
+      const float gm = sqrtf(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+      alpha *= use_galpha ? tf1d_lookup(t_galpha, n, gmult * (gm - glo)) : 1.0f;
+      const float pl = satf(fabsf(gr.x * inc.x * sx + gr.y * inc.y * sy + gr.z * inc.z * sz) / (gm * rlen));
+      shd = amb + dif * pl;
+      shs = spc * powf(pl, spw);
+    } else {
+      shd = amb;
+      shs = 0.0f;
+    }
+    const float m = Tr * alpha;
+    Tr *= (1.0f - alpha);
+    cr += m * satf(shd * tf1d_lookup(t_r, n, u) + shs);
+    cg += m * satf(shd * tf1d_lookup(t_g, n, u) + shs);
+    cb += m * satf(shd * tf1d_lookup(t_b, n, u) + shs);
+  };
+
+  while (k > 0) {
+    // the 1D caster compares unconverted exclusion bounds (V/...1D...cuh:74)
+    if (es > k && ee < k) {
+      p.x += (float)(k - ee) * inc.x; p.y += (float)(k - ee) * inc.y; p.z += (float)(k - ee) * inc.z;
+      k = ee;
+      sa = 0;
+      sb = 0;
+      continue;
+    }
+    const V3<float> q = cur_pos<PRECISE>(p, p0, inc, kst, k);
+    const float u = mult * (trilinear(vol, g, q.x, q.y, q.z) - lo);
+    const float alpha = tf1d_lookup(t_alpha, n, u);
+    if (SKIP) {
+      if (alpha > 0.0f) {
+        sa = sb;
+        sb = 0;
+        if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
+        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
+        if (!sa) accumulate(alpha, u);
+        if (Tr < term) { Tr = 0.0f; break; }
+      } else {
+        if (!sa) { step(p, inc); k--; }
+        sb = !sa;
+        step(p, inc);
+        k--;
+        sa = 0;
+      }
+    } else {
+      step(p, inc);
+      k--;
+      if (alpha > 0.0f) {
+        accumulate(alpha, u);
+        if (Tr < term) { Tr = 0.0f; break; }
+      }
+    }
+  }
+  const long idx = (long)y * a.v.W + x;
+  const uint32_t R = to_u8(255.0f * satf(cr)), G = to_u8(255.0f * satf(cg)), B = to_u8(255.0f * satf(cb));
+  const uint32_t A = to_u8(255.0f * (1.0f - Tr));
+  a.out[idx] = R | (G << 8) | (B << 16) | (A << 24);
+  if (a.depth) a.depth[idx] = (float)(k0 - k);
+  if (a.min_depth) a.min_depth[idx] = r.min_depth;
+  if (a.max_depth) a.max_depth[idx] = r.max_depth;
+}
+
+// ===========================================================================
+// 2D transfer function
+struct Cast2dArgs {
+  VolGeom g;
+  ViewGeom v;
+  float step_len, term_T;
+  int jitter;
+  int n;
+  rvk_tf2d_params p;
+  const Tf2dCell* cells;
+  const Tf2dCell* kcells;
+  const void* vol;
+  const uint8_t* gradmag;
+  float gradmag_scale;
+  uint32_t* out;
+  float* depth;
+  float* min_depth;
+  float* max_depth;
+  int tiles_x;
+};
+
+// V/...2D...cuh:30-351.  DUAL selects the keyhole-TF variant (WithDual).
+// GMAG reads |grad| from the precomputed u8 volume instead of 6 extra gathers;
+// without a direction the Phong-Lambert factor is taken as 1.
+template <typename VT, bool SKIP, bool DUAL, bool GMAG, bool PRECISE>
+__global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  int x, y;
+  tile_pixel(tile, a.tiles_x, x, y);
+  if (x >= a.v.W || y >= a.v.H) return;
+  const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
+  const VT* __restrict__ vol = (const VT*)a.vol;
+  const VolGeom& g = a.g;
+  const int n = a.n;
+
+  float Tr = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  const float lo = a.p.intensity_low, mult = a.p.intensity_multiplier;
+  const float glo = a.p.gradient_low, gmult = a.p.gradient_multiplier, goff = a.p.gradient_offset;
+  const float term = a.term_T;
+  const float j = ray_offset(a.jitter, x, y);
+  int k = step_count(r.num_steps - j);
+  const int k0 = k;
+  V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  const V3<float> inc = r.inc;
+  const float rlen = ray_length(g, inc);
+  const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
+  // WithDual rounds the exclusion bounds up/down, WithoutDual down/up
+  int es = DUAL ? (int)ceilf(r.exclude_start) : (int)floorf(r.exclude_start);
+  int ee = DUAL ? (int)floorf(r.exclude_end) : (int)ceilf(r.exclude_end);
+  es = k - es;
+  ee = k - ee;
+  const int kst = k;
+  const V3<float> p0 = p;
+  int sa = 0, sb = 0;
+  while (k > 0) {
+    const bool in_key = (es >= k && ee < k);
+    if (!DUAL && in_key) {
+      p.x += (float)(k - ee) * inc.x; p.y += (float)(k - ee) * inc.y; p.z += (float)(k - ee) * inc.z;
+      k = ee;
+      sa = 0;
+      sb = 0;
+      continue;
+    }
+    const V3<float> q = cur_pos<PRECISE>(p, p0, inc, kst, k);
+    const float u = mult * (trilinear(vol, g, q.x, q.y, q.z) - lo);
+    V3<float> gr{0.0f, 0.0f, 0.0f};
+    float gidx, gmag;
+    if (GMAG) {
+      gmag = trilinear(a.gradmag, g, q.x, q.y, q.z) * a.gradmag_scale;
+      gidx = gmult * (log2f(gmag * gmag + goff) + glo);
+    } else {
+      gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
+      gr.y = (trilinear(vol, g, q.x, q.y + 1.0f, q.z) - trilinear(vol, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
+      gr.z = (trilinear(vol, g, q.x, q.y, q.z + 1.0f) - trilinear(vol, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
+      const float gm2 = gr.x * gr.x + gr.y * gr.y + gr.z * gr.z;
+      gidx = gmult * (log2f(gm2 + goff) + glo);
+      gmag = sqrtf(gm2);
+    }
+    const int ti = tf2d_index(n, u, gidx);
+    const Tf2dCell* __restrict__ cells = (DUAL && in_key) ? a.kcells : a.cells;
+    const float4 c0 = *(const float4*)&cells[ti].alpha;
+    const float alpha = c0.x;
+    auto accumulate = [&]() {
+      const float4 c1 = *(const float4*)&cells[ti].spec_power;
+      const float pl = GMAG ? 1.0f
+                            : satf(fabsf(gr.x * inc.x * sx + gr.y * inc.y * sy + gr.z * inc.z * sz) / (gmag * rlen));
+      const float shd = c0.y + c0.z * pl;
+      const float shs = c0.w * powf(pl, c1.x);
+      const float m = Tr * alpha;
+      Tr *= (1.0f - alpha);
+      cr += m * satf(shd * c1.y + shs);
+      cg += m * satf(shd * c1.z + shs);
+      cb += m * satf(shd * c1.w + shs);
+    };
+    if (SKIP) {
+      if (alpha > 0.0f) {
+        sa = sb;
+        sb = 0;
+        if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
+        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
+        if (!sa) accumulate();
+        if (Tr < term) { Tr = 0.0f; break; }
+      } else {
+        if (!sa) { step(p, inc); k--; }
+        sb = !sa;
+        step(p, inc);
+        k--;
+        sa = 0;
+      }
+    } else {
+      step(p, inc);
+      k--;
+      if (alpha > 0.0f) {
+        accumulate();
+        if (Tr < term) { Tr = 0.0f; break; }
+      }
+    }
+  }
+  const long idx = (long)y * a.v.W + x;
+  const uint32_t R = to_u8(255.0f * cr), G = to_u8(255.0f * cg), B = to_u8(255.0f * cb);
+  const uint32_t A = to_u8(255.0f * (1.0f - Tr));
+  a.out[idx] = R | (G << 8) | (B << 16) | (A << 24);
+  if (a.depth) a.depth[idx] = (float)(k0 - k);
+  if (a.min_depth) a.min_depth[idx] = r.min_depth;
+  if (a.max_depth) a.max_depth[idx] = r.max_depth;
+}
+
+// ===========================================================================
+// Post passes (V/...renderAlgo.cu:449-512), fused into one kernel per pixel.
+// The depth normalisation of a pixel's neighbours is recomputed locally, so
+// no pass ordering between pixels is needed.
+__global__ __launch_bounds__(256) void post_shade_kernel(int W, int H, rvk_shade_params s, const float* depth,
+                                                         const float* mind, const float* maxd, uint32_t* rgba,
+                                                         float* depth_out) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= W || y >= H) return;
+  auto norm = [&](long i) {
+    const float c = depth[i], mx = maxd[i], mn = mind[i];
+    return (mx > 0.0f) ? (c + mn) / mx : 1.0f;
+  };
+  const long i = (long)y * W + x;
+  const float d0 = norm(i);
+  const float dy = norm((long)min(y + 1, H - 1) * W + x);
+  const float dx = norm((long)y * W + min(x + 1, W - 1));
+  float gm = sqrt_rz((dx - d0) * (dx - d0) + (dy - d0) * (dy - d0));
+  gm = 1.0f - s.celr * satf((gm - s.cela) * s.celc);
+  gm *= 1.0f - s.disr * satf((d0 - s.disa) * s.disc);
+  const uint32_t c = rgba[i];
+  const uint32_t R = to_u8(gm * (float)(c & 255u)), G = to_u8(gm * (float)((c >> 8) & 255u));
+  const uint32_t B = to_u8(gm * (float)((c >> 16) & 255u));
+  rgba[i] = R | (G << 8) | (B << 16) | (c & 0xff000000u);
+  if (depth_out) depth_out[i] = d0;
+}
+
+// ===========================================================================
+// Ray-sample count of the plain march without opacity termination: the sum
+// over pixels of floor(num_steps - offset).  This is the DRR's metric
+// denominator; it depends only on geometry, not on voxel values.
+__global__ __launch_bounds__(256) void count_steps_kernel(VolGeom g, ViewGeom v, float step_len, int jitter,
+                                                          int tiles_x, unsigned long long* out) {
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  int x, y;
+  tile_pixel(tile, tiles_x, x, y);
+  unsigned long long c = 0;
+  if (x < v.W && y < v.H) {
+    const Ray r = form_ray(g, v, step_len, x, y);
+    c = (unsigned long long)step_count(r.num_steps - ray_offset(jitter, x, y));
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+// ===========================================================================
+template <typename F> static int dispatch_vt(int dtype, F&& f) {
+  switch (dtype) {
+    case RVK_U8: return f((uint8_t*)nullptr);
+    case RVK_I8: return f((int8_t*)nullptr);
+    case RVK_U16: return f((uint16_t*)nullptr);
+    case RVK_I16: return f((int16_t*)nullptr);
+    case RVK_U32: return f((uint32_t*)nullptr);
+    case RVK_I32: return f((int32_t*)nullptr);
+    case RVK_F32: return f((float*)nullptr);
+    default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+  }
+}
+
+static int check_march(const rvk_march_params* m) {
+  if (!m) return fail(RVK_ERR_INVALID_ARGUMENT, "march params are NULL");
+  if (!(m->step_scale > 0.0f) || !std::isfinite(m->step_scale))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "step_scale must be positive and finite");
+  return RVK_OK;
+}
+
+static int launch_grid(const rvk_view* v, int& tiles_x) {
+  tiles_x = (v->width + 15) / 16;
+  return tiles_x * ((v->height + 15) / 16);
+}
+
+}  // namespace rvk
+
+using namespace rvk;
+
+extern "C" {
+
+int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_view* view, const rvk_march_params* march,
+                   const rvk_drr_params* drr, void* out) {
+  if (!ctx || !drr || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = check_frame(vol, frame);
+  if (!rc) rc = check_view(view);
+  if (!rc) rc = check_march(march);
+  if (rc) return rc;
+  if (drr->output < RVK_DRR_RGBA8_REF || drr->output > RVK_DRR_F32_L)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown DRR output mode %d", drr->output);
+  if (!(drr->ct_offset > 0.0f)) return fail(RVK_ERR_INVALID_ARGUMENT, "ct_offset must be positive");
+  RVK_HIP(hipSetDevice(ctx->device));
+  DrrArgs a;
+  a.g = vol_geom(vol);
+  a.v = view_geom(view);
+  a.step_len = min_spacing(vol) * march->step_scale;
+  a.jitter = march->jitter;
+  a.skip = march->skip_rule;
+  a.ref_term = drr->ref_termination;
+  a.intercept = drr->ct_intercept;
+  a.slope = drr->ct_slope;
+  a.offset = drr->ct_offset;
+  a.output = drr->output;
+  a.tint = (uint32_t)drr->tint[0] | ((uint32_t)drr->tint[1] << 8) | ((uint32_t)drr->tint[2] << 16) |
+           ((uint32_t)drr->tint[3] << 24);
+  a.vol = vol->frames[frame];
+  a.out = out;
+  const int nblk = launch_grid(view, a.tiles_x);
+  return dispatch_vt(vol->dtype, [&](auto tag) {
+    using VT = std::remove_pointer_t<decltype(tag)>;
+    const bool pr = march->precise != 0;
+    if (a.skip && pr) hipLaunchKernelGGL((drr_kernel<VT, true, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else if (a.skip) hipLaunchKernelGGL((drr_kernel<VT, true, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else if (pr) hipLaunchKernelGGL((drr_kernel<VT, false, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((drr_kernel<VT, false, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    RVK_HIP(hipGetLastError());
+    return (int)RVK_OK;
+  });
+}
+
+int rvk_render_1d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf1d* tf, const rvk_view* view,
+                  const rvk_march_params* march, uint8_t* out, const rvk_aux_out* aux) {
+  if (!ctx || !tf || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = check_frame(vol, frame);
+  if (!rc) rc = check_view(view);
+  if (!rc) rc = check_march(march);
+  if (rc) return rc;
+  RVK_HIP(hipSetDevice(ctx->device));
+  Cast1dArgs a;
+  a.g = vol_geom(vol);
+  a.v = view_geom(view);
+  a.step_len = min_spacing(vol) * march->step_scale;
+  a.term_T = march->term_T;
+  a.jitter = march->jitter;
+  a.n = tf->n;
+  a.p = tf->p;
+  a.tables = tf->tables;
+  a.vol = vol->frames[frame];
+  a.out = (uint32_t*)out;
+  a.depth = aux ? aux->depth : nullptr;
+  a.min_depth = aux ? aux->min_depth : nullptr;
+  a.max_depth = aux ? aux->max_depth : nullptr;
+  const int nblk = launch_grid(view, a.tiles_x);
+  const size_t lds = sizeof(float) * 5 * (size_t)tf->n;
+  const bool grad = !(tf->p.diffuse == 0.0f && tf->p.specular == 0.0f && tf->p.specular_power >= 0.0f &&
+                      !std::isfinite(tf->p.gradient_multiplier));
+  const bool skip = march->skip_rule != 0, pr = march->precise != 0;
+  return dispatch_vt(vol->dtype, [&](auto tag) {
+    using VT = std::remove_pointer_t<decltype(tag)>;
+#define RVK_L1D(S, G, P) \
+  hipLaunchKernelGGL((cast1d_kernel<VT, S, G, P>), dim3(nblk), dim3(256), lds, ctx->stream, a)
+    if (pr) {
+      if (skip) { if (grad) RVK_L1D(true, true, true); else RVK_L1D(true, false, true); }
+      else { if (grad) RVK_L1D(false, true, true); else RVK_L1D(false, false, true); }
+    } else {
+      if (skip) { if (grad) RVK_L1D(true, true, false); else RVK_L1D(true, false, false); }
+      else { if (grad) RVK_L1D(false, true, false); else RVK_L1D(false, false, false); }
+    }
+#undef RVK_L1D
+    RVK_HIP(hipGetLastError());
+    return (int)RVK_OK;
+  });
+}
+
+int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d* tf, const rvk_view* view,
+                  const rvk_march_params* march, int use_gradmag, uint8_t* out, const rvk_aux_out* aux) {
+  if (!ctx || !tf || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = check_frame(vol, frame);
+  if (!rc) rc = check_view(view);
+  if (!rc) rc = check_march(march);
+  if (rc) return rc;
+  if (use_gradmag && !vol->gradmag[frame])
+    return fail(RVK_ERR_NOT_READY, "use_gradmag set but frame %d has no gradient-magnitude volume", frame);
+  RVK_HIP(hipSetDevice(ctx->device));
+  Cast2dArgs a;
+  a.g = vol_geom(vol);
+  a.v = view_geom(view);
+  a.step_len = min_spacing(vol) * march->step_scale;
+  a.term_T = march->term_T;
+  a.jitter = march->jitter;
+  a.n = tf->n;
+  a.p = tf->p;
+  a.cells = tf->cells;
+  a.kcells = tf->kcells;
+  a.vol = vol->frames[frame];
+  a.gradmag = use_gradmag ? vol->gradmag[frame] : nullptr;
+  a.gradmag_scale = use_gradmag ? vol->gradmag_scale[frame] : 0.0f;
+  a.out = (uint32_t*)out;
+  a.depth = aux ? aux->depth : nullptr;
+  a.min_depth = aux ? aux->min_depth : nullptr;
+  a.max_depth = aux ? aux->max_depth : nullptr;
+  const int nblk = launch_grid(view, a.tiles_x);
+  const bool skip = march->skip_rule != 0, dual = tf->kcells != nullptr, gm = use_gradmag != 0;
+  const bool pr = march->precise != 0;
+  return dispatch_vt(vol->dtype, [&](auto tag) {
+    using VT = std::remove_pointer_t<decltype(tag)>;
+#define RVK_L2D(S, D, G)                                                                              \
+  do {                                                                                                \
+    if (pr) hipLaunchKernelGGL((cast2d_kernel<VT, S, D, G, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);  \
+    else hipLaunchKernelGGL((cast2d_kernel<VT, S, D, G, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);    \
+  } while (0)
+    if (skip) {
+      if (dual) { if (gm) RVK_L2D(true, true, true); else RVK_L2D(true, true, false); }
+      else { if (gm) RVK_L2D(true, false, true); else RVK_L2D(true, false, false); }
+    } else {
+      if (dual) { if (gm) RVK_L2D(false, true, true); else RVK_L2D(false, true, false); }
+      else { if (gm) RVK_L2D(false, false, true); else RVK_L2D(false, false, false); }
+    }
+#undef RVK_L2D
+    RVK_HIP(hipGetLastError());
+    return (int)RVK_OK;
+  });
+}
+
+int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
+                   const float* min_depth, const float* max_depth, uint8_t* rgba) {
+  if (!ctx || !shade || !depth || !min_depth || !max_depth || !rgba)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (width <= 0 || height <= 0) return fail(RVK_ERR_INVALID_ARGUMENT, "bad image size");
+  RVK_HIP(hipSetDevice(ctx->device));
+  dim3 grid((width + 15) / 16, (height + 15) / 16);
+  // depth is normalised in place only after every pixel has read its
+  // neighbours, so the normalised copy goes to a second buffer first.
+  float* tmp = nullptr;
+  RVK_HIP(hipMallocAsync((void**)&tmp, sizeof(float) * (size_t)width * height, ctx->stream));
+  hipLaunchKernelGGL(post_shade_kernel, grid, dim3(256), 0, ctx->stream, width, height, *shade, depth, min_depth,
+                     max_depth, (uint32_t*)rgba, tmp);
+  RVK_HIP(hipGetLastError());
+  RVK_HIP(hipMemcpyAsync(depth, tmp, sizeof(float) * (size_t)width * height, hipMemcpyDeviceToDevice, ctx->stream));
+  RVK_HIP(hipFreeAsync(tmp, ctx->stream));
+  return RVK_OK;
+}
+
+}  // extern "C"
+
+extern "C" int rvk_count_ray_steps(rvk_ctx* ctx, const rvk_volume* vol, const rvk_view* view,
+                                   const rvk_march_params* march, int64_t* out) {
+  if (!ctx || !vol || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = check_view(view);
+  if (!rc) rc = check_march(march);
+  if (rc) return rc;
+  RVK_HIP(hipSetDevice(ctx->device));
+  rc = ensure_scratch(ctx, 4096);
+  if (rc) return rc;
+  unsigned long long* d = (unsigned long long*)ctx->scratch;
+  RVK_HIP(hipMemsetAsync(d, 0, sizeof(*d), ctx->stream));
+  int tiles_x;
+  const int nblk = launch_grid(view, tiles_x);
+  hipLaunchKernelGGL(count_steps_kernel, dim3(nblk), dim3(256), 0, ctx->stream, vol_geom(vol), view_geom(view),
+                     min_spacing(vol) * march->step_scale, march->jitter, tiles_x, d);
+  RVK_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  RVK_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  RVK_HIP(hipStreamSynchronize(ctx->stream));
+  *out = (int64_t)h;
+  return RVK_OK;
+}

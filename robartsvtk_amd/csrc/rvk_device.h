@@ -1,0 +1,304 @@
+// rvk_device.h -- device-side building blocks shared by the gfx950 casters.
+//
+// Semantics follow the reference kernels (paths relative to the reference
+// root, V/ = Visualization/vtkCudaVisualization):
+//   ray setup .......... V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:38-447
+//   volume texture ..... unnormalised / linear / clamp (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:305-316)
+//   1D TF texture ...... normalised / linear / clamp  (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:248-267)
+//   2D TF texture ...... normalised / point / clamp   (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:21-25)
+// The texture unit is replaced by explicit gathers from a native-type,
+// 4x4x4-bricked volume with exact FP32 weights.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rvk {
+
+// ---------------------------------------------------------------------------
+// Parameters every caster receives by value (no __constant__ module state).
+struct VolGeom {
+  int n[3];        // dims
+  int bys, bzs;    // brick-row / brick-slab strides in elements (64*nbx, 64*nbx*nby)
+  float srecip[3]; // 0.5/spacing  (V/vtkCudaVolumeInformationHandler.cxx:91-93)
+  float sp[3];     // spacing
+  float bounds[6]; // [0, dim-1] per axis
+};
+
+struct ViewGeom {
+  float m[16];
+  int W, H, flipped;
+  int n_clip, n_key;
+  float clip[24];
+  float key[24];
+  const float* zbuf;
+  int zbw, zbh;
+};
+
+struct MarchArgs {
+  float step_len;  // MinSpacing * step_scale
+  float term_T;
+  int jitter;
+};
+
+template <typename T> struct V3 { T x, y, z; };
+
+__device__ __forceinline__ float satf(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+// __fsqrt_ru / __fsqrt_rz from the correctly rounded sqrtf and the sign of the
+// exact FMA residual.
+__device__ __forceinline__ float sqrt_ru(float x) {
+  float r = sqrtf(x);
+  if (fmaf(r, r, -x) < 0.0f) r = nextafterf(r, __builtin_huge_valf());
+  return r;
+}
+__device__ __forceinline__ float sqrt_rz(float x) {
+  float r = sqrtf(x);
+  if (fmaf(r, r, -x) > 0.0f) r = nextafterf(r, 0.0f);
+  return r;
+}
+
+// Steps a ray takes: floor(num_steps - offset) as __float2int_rd gives it for
+// any real ray.  Non-finite or absurd counts (>= 2^24 steps) march nothing,
+// so no input geometry can keep a wave looping without bound.
+__device__ __forceinline__ int step_count(float v) { return (v > 0.0f && v < 16777216.0f) ? (int)floorf(v) : 0; }
+
+// CUDA float->uchar conversion: truncate toward zero, saturate, NaN -> 0.
+__device__ __forceinline__ uint32_t to_u8(float v) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 255.0f) return 255u;
+  return (uint32_t)(int)v;
+}
+
+// ---------------------------------------------------------------------------
+// Bricked addressing: voxel (i,j,k) lives at brick (i>>2, j>>2, k>>2), offset
+// (i&3) + 4*(j&3) + 16*(k&3) inside a 64-voxel brick.  The address separates
+// per axis, so the 8 corners of a trilinear cell cost 6 axis terms + 8 adds.
+__device__ __forceinline__ int brick_x(int i) { return ((i >> 2) << 6) | (i & 3); }
+__device__ __forceinline__ int brick_y(int j, int bys) { return (j >> 2) * bys + ((j & 3) << 2); }
+__device__ __forceinline__ int brick_z(int k, int bzs) { return (k >> 2) * bzs + ((k & 3) << 4); }
+
+template <typename VT> __device__ __forceinline__ float ldv(const VT* __restrict__ d, int off) {
+  return (float)d[off];
+}
+
+// tex3D(unnormalised, linear, clamp) at p: trilinear at p-0.5 with indices
+// clamped to [0, n-1].  Clamping the coordinate to [-1, n] first leaves every
+// clamp-addressed result unchanged and keeps the integer conversion in range.
+template <typename VT>
+__device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGeom& g, float px, float py, float pz) {
+  float x = px - 0.5f, y = py - 0.5f, z = pz - 0.5f;
+  x = fminf(fmaxf(x, -1.0f), (float)g.n[0]);
+  y = fminf(fmaxf(y, -1.0f), (float)g.n[1]);
+  z = fminf(fmaxf(z, -1.0f), (float)g.n[2]);
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const float ax = x - fx, ay = y - fy, az = z - fz;
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  const int x0 = min(max(ix, 0), g.n[0] - 1), x1 = min(max(ix + 1, 0), g.n[0] - 1);
+  const int y0 = min(max(iy, 0), g.n[1] - 1), y1 = min(max(iy + 1, 0), g.n[1] - 1);
+  const int z0 = min(max(iz, 0), g.n[2] - 1), z1 = min(max(iz + 1, 0), g.n[2] - 1);
+  const int ox0 = brick_x(x0), ox1 = brick_x(x1);
+  const int oy0 = brick_y(y0, g.bys), oy1 = brick_y(y1, g.bys);
+  const int oz0 = brick_z(z0, g.bzs), oz1 = brick_z(z1, g.bzs);
+  const float v000 = ldv(d, ox0 + oy0 + oz0), v100 = ldv(d, ox1 + oy0 + oz0);
+  const float v010 = ldv(d, ox0 + oy1 + oz0), v110 = ldv(d, ox1 + oy1 + oz0);
+  const float v001 = ldv(d, ox0 + oy0 + oz1), v101 = ldv(d, ox1 + oy0 + oz1);
+  const float v011 = ldv(d, ox0 + oy1 + oz1), v111 = ldv(d, ox1 + oy1 + oz1);
+  const float c00 = v000 + ax * (v100 - v000);
+  const float c10 = v010 + ax * (v110 - v010);
+  const float c01 = v001 + ax * (v101 - v001);
+  const float c11 = v011 + ax * (v111 - v011);
+  const float c0 = c00 + ay * (c10 - c00);
+  const float c1 = c01 + ay * (c11 - c01);
+  return c0 + az * (c1 - c0);
+}
+
+// tex1D(normalised, linear, clamp) on an n-entry table.
+__device__ __forceinline__ float tf1d_lookup(const float* t, int n, float u) {
+  float x = u * (float)n - 0.5f;
+  x = fminf(fmaxf(x, -1.0f), (float)n);
+  const float fx = floorf(x);
+  const float a = x - fx;
+  const int i = (int)fx;
+  const int i0 = min(max(i, 0), n - 1), i1 = min(max(i + 1, 0), n - 1);
+  const float v0 = t[i0], v1 = t[i1];
+  return v0 + a * (v1 - v0);
+}
+
+// tex2D(normalised, point, clamp) cell index g*n + i.
+__device__ __forceinline__ int tf2d_index(int n, float u, float v) {
+  const float x = fminf(fmaxf(u * (float)n, -1.0f), (float)n);
+  const float y = fminf(fmaxf(v * (float)n, -1.0f), (float)n);
+  const int i = min(max((int)floorf(x), 0), n - 1);
+  const int j = min(max((int)floorf(y), 0), n - 1);
+  return j * n + i;
+}
+
+// ---------------------------------------------------------------------------
+// Ray setup (formRays, fused into every caster's prologue).
+struct Ray {
+  V3<float> start, inc;
+  float num_steps;
+  float exclude_start, exclude_end;
+  float min_depth, max_depth;
+};
+
+// V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:130-197
+__device__ __forceinline__ void clip_planes(const ViewGeom& v, V3<float>& s, V3<float>& e, V3<float>& d) {
+  const int np = v.n_clip;
+  if (!np) return;
+  int flag = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if (i >= np) break;
+    d.x = e.x - s.x; d.y = e.y - s.y; d.z = e.z - s.z;
+    const float px = v.clip[4 * i], py = v.clip[4 * i + 1], pz = v.clip[4 * i + 2], pw = v.clip[4 * i + 3];
+    const float dp = px * d.x + py * d.y + pz * d.z;
+    const float t = -(px * s.x + py * s.y + pz * s.z + pw) / dp;
+    const float p0 = s.x + t * d.x, p1 = s.y + t * d.y, p2 = s.z + t * d.z;
+    if (t > 0.0f && t < 1.0f) {
+      if (dp > 0.0f) { s.x = p0; s.y = p1; s.z = p2; }
+      else { e.x = p0; e.y = p1; e.z = p2; }
+    }
+    flag |= (dp > 0.0f && t > 1.0f);
+    flag |= (dp < 0.0f && t < 0.0f);
+  }
+  if (flag) s = e;
+}
+
+// V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:199-318
+__device__ __forceinline__ void clip_volume(const VolGeom& g, V3<float>& s, V3<float>& e, V3<float>& d) {
+  d.x = e.x - s.x; d.y = e.y - s.y; d.z = e.z - s.z;
+  const float b0 = g.bounds[0] + 1.0f, b1 = g.bounds[1] - 1.0f;
+  const float b2 = g.bounds[2] + 1.0f, b3 = g.bounds[3] - 1.0f;
+  const float b4 = g.bounds[4] + 1.0f, b5 = g.bounds[5] - 1.0f;
+  float ds, de;
+#define RVK_CROP()                                                   \
+  if (isfinite(ds)) {                                                \
+    s.x += d.x * ds; s.y += d.y * ds; s.z += d.z * ds;               \
+    e.x += d.x * de; e.y += d.y * de; e.z += d.z * de;               \
+  }
+  if (d.x > 0.0f) { ds = s.x < b0 ? b0 - s.x : 0.0f; de = e.x > b1 ? b1 - e.x : 0.0f; }
+  else { ds = s.x > b1 ? b1 - s.x : 0.0f; de = e.x < b0 ? b0 - e.x : 0.0f; }
+  ds /= d.x; de /= d.x;
+  RVK_CROP()
+  if (d.y > 0.0f) { ds = s.y < b2 ? b2 - s.y : 0.0f; de = e.y > b3 ? b3 - e.y : 0.0f; }
+  else { ds = s.y > b3 ? b3 - s.y : 0.0f; de = e.y < b2 ? b2 - e.y : 0.0f; }
+  ds /= d.y; de /= d.y;
+  RVK_CROP()
+  if (d.z > 0.0f) { ds = s.z < b4 ? b4 - s.z : 0.0f; de = e.z > b5 ? b5 - e.z : 0.0f; }
+  else { ds = s.z > b5 ? b5 - s.z : 0.0f; de = e.z < b4 ? b4 - e.z : 0.0f; }
+  ds /= d.z; de /= d.z;
+  RVK_CROP()
+#undef RVK_CROP
+  if (e.x > b1 + 1.0f || e.y > b3 + 1.0f || e.z > b5 + 1.0f || e.x < b0 - 1.0f || e.y < b2 - 1.0f ||
+      e.z < b4 - 1.0f || s.x > b1 + 1.0f || s.y > b3 + 1.0f || s.z > b5 + 1.0f || s.x < b0 - 1.0f ||
+      s.y < b2 - 1.0f || s.z < b4 - 1.0f) {
+    s = e;
+  }
+  d.x = e.x - s.x; d.y = e.y - s.y; d.z = e.z - s.z;
+}
+
+// V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:38-128
+__device__ __forceinline__ void keyhole_values(const ViewGeom& v, V3<float> s, const V3<float>& inc, float ns,
+                                               float& es, float& ee) {
+  es = 1.0f;
+  ee = -1.0f;
+  const int np = v.n_key;
+  if (!np) return;
+  const V3<float> old = s;
+  V3<float> d{ns * inc.x, ns * inc.y, ns * inc.z};
+  V3<float> e{s.x + d.x, s.y + d.y, s.z + d.z};
+  int flag = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if (i >= np) break;
+    d.x = e.x - s.x; d.y = e.y - s.y; d.z = e.z - s.z;
+    const float px = v.key[4 * i], py = v.key[4 * i + 1], pz = v.key[4 * i + 2], pw = v.key[4 * i + 3];
+    const float dp = px * d.x + py * d.y + pz * d.z;
+    const float t = -(px * s.x + py * s.y + pz * s.z + pw) / dp;
+    const float p0 = s.x + t * d.x, p1 = s.y + t * d.y, p2 = s.z + t * d.z;
+    if (t > 0.0f && t < 1.0f) {
+      if (dp > 0.0f) { s.x = p0; s.y = p1; s.z = p2; }
+      else { e.x = p0; e.y = p1; e.z = p2; }
+    }
+    flag |= (dp > 0.0f && t > 1.0f);
+    flag |= (dp < 0.0f && t < 0.0f);
+  }
+  s.x -= old.x; s.y -= old.y; s.z -= old.z;
+  e.x -= old.x; e.y -= old.y; e.z -= old.z;
+  const float inv = 1.0f / (inc.x * inc.x + inc.y * inc.y + inc.z * inc.z);
+  es = flag ? -1.0f : (s.x * inc.x + s.y * inc.y + s.z * inc.z) * inv;
+  ee = flag ? -1.0f : (e.x * inc.x + e.y * inc.y + e.z * inc.z) * inv;
+}
+
+// V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:320-447
+__device__ __forceinline__ Ray form_ray(const VolGeom& g, const ViewGeom& v, float step_len, int x, int y) {
+  Ray r;
+  const float vx = v.flipped ? (float)x / (float)v.W : 1.0f - (float)x / (float)v.W;
+  const float vy = (float)y / (float)v.H;
+  float end_depth = 1.0f;
+  if (v.zbuf) {
+    const float u = 1.0f - vx;
+    int zi = (int)floorf(fminf(fmaxf(u * (float)v.zbw, -1.0f), (float)v.zbw));
+    int zj = (int)floorf(fminf(fmaxf(vy * (float)v.zbh, -1.0f), (float)v.zbh));
+    zi = min(max(zi, 0), v.zbw - 1);
+    zj = min(max(zj, 0), v.zbh - 1);
+    end_depth = v.zbuf[(long)zj * v.zbw + zi];
+  }
+  const float* m = v.m;
+  V3<float> s, e, f;
+  s.x = vx * m[0] + vy * m[1] + m[3];
+  s.y = vx * m[4] + vy * m[5] + m[7];
+  s.z = vx * m[8] + vy * m[9] + m[11];
+  const float sn = vx * m[12] + vy * m[13] + m[15];
+  e.x = s.x + end_depth * m[2];
+  e.y = s.y + end_depth * m[6];
+  e.z = s.z + end_depth * m[10];
+  const float en = sn + end_depth * m[14];
+  f.x = s.x + m[2];
+  f.y = s.y + m[6];
+  f.z = s.z + m[10];
+  const float fn = sn + m[14];
+  s.x /= sn; s.y /= sn; s.z /= sn;
+  e.x /= en; e.y /= en; e.z /= en;
+  f.x /= fn; f.y /= fn; f.z /= fn;
+  const V3<float> old = s;
+  V3<float> d{f.x - s.x, f.y - s.y, f.z - s.z};
+  r.max_depth = sqrt_rz(d.x * d.x + d.y * d.y + d.z * d.z);
+  clip_planes(v, s, e, d);
+  clip_volume(g, s, e, d);
+  const float len = sqrt_rz(d.x * d.x + d.y * d.y + d.z * d.z);
+  const float md = sqrt_rz((s.x - old.x) * (s.x - old.x) + (s.y - old.y) * (s.y - old.y) +
+                           (s.z - old.z) * (s.z - old.z));
+  r.min_depth = (len > 0.0f) ? md : r.max_depth;
+  r.num_steps = sqrt_ru(d.x * d.x * g.sp[0] * g.sp[0] + d.y * d.y * g.sp[1] * g.sp[1] +
+                        d.z * d.z * g.sp[2] * g.sp[2]) / step_len;
+  r.inc.x = d.x / r.num_steps;
+  r.inc.y = d.y / r.num_steps;
+  r.inc.z = d.z / r.num_steps;
+  r.start = s;
+  keyhole_values(v, s, r.inc, r.num_steps, r.exclude_start, r.exclude_end);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Pixel tiling: a 256-thread workgroup covers a 16x16 pixel tile, each wave64
+// an 8x8 quadrant.  Workgroups are dispatched round-robin over the 8 XCDs;
+// the remap gives each XCD a contiguous run of tiles (whole tile rows), so
+// rays sharing volume bricks share that XCD's L2.
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int per = nblk / kNumXcd, rem = nblk % kNumXcd;
+  const int xcd = bid % kNumXcd, local = bid / kNumXcd;
+  return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+__device__ __forceinline__ void tile_pixel(int tile, int tiles_x, int& px, int& py) {
+  const int t = threadIdx.x;
+  const int w = t >> 6, l = t & 63;
+  px = (tile % tiles_x) * 16 + (w & 1) * 8 + (l & 7);
+  py = (tile / tiles_x) * 16 + (w >> 1) * 8 + (l >> 3);
+}
+
+}  // namespace rvk

@@ -1,0 +1,478 @@
+// rvk_runtime.hip -- contexts, volumes, transfer functions and errors.
+//
+// Replaces, on HIP:
+//   C/CudaObject.cxx:55-212 and C/vtkCudaDeviceManager.cxx:50-443 (device and
+//     stream per object; here one rvk_ctx = one device + one stream);
+//   V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:584-651 (CUDA_castBuffer: host
+//     copy + widening to float) and the *_loadImageInfo 3D-array uploads
+//     (e.g. V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:409-443): volumes stay
+//     in their native type and are relaid out into 4x4x4 bricks on the device;
+//   the TF loadTextures functions (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:324-372,
+//     V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:589-651).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "rvk_internal.h"
+
+namespace rvk {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  const int st = (e == hipErrorOutOfMemory) ? RVK_ERR_OUT_OF_MEMORY
+                 : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? RVK_ERR_NO_DEVICE
+                                                                          : RVK_ERR_HIP;
+  return fail(st, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case RVK_U8: case RVK_I8: return 1;
+    case RVK_U16: case RVK_I16: return 2;
+    case RVK_U32: case RVK_I32: case RVK_F32: return 4;
+    default: return 0;
+  }
+}
+
+// V/vtkCudaVolumeInformationHandler.cxx:80-108
+VolGeom vol_geom(const rvk_volume* v) {
+  VolGeom g;
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = v->dims[a];
+    g.srecip[a] = (float)(0.5 / v->spacing[a]);
+    g.sp[a] = (float)v->spacing[a];
+    g.bounds[2 * a] = 0.0f;
+    g.bounds[2 * a + 1] = (float)v->dims[a] - 1.0f;
+  }
+  g.bys = 64 * v->nb[0];
+  g.bzs = 64 * v->nb[0] * v->nb[1];
+  return g;
+}
+
+float min_spacing(const rvk_volume* v) {
+  return (float)std::fmin(v->spacing[0], std::fmin(v->spacing[1], v->spacing[2]));
+}
+
+ViewGeom view_geom(const rvk_view* v) {
+  ViewGeom g;
+  std::memcpy(g.m, v->view_to_voxels, sizeof(g.m));
+  g.W = v->width;
+  g.H = v->height;
+  g.flipped = v->flipped;
+  g.n_clip = v->n_clip_planes;
+  g.n_key = v->n_keyhole_planes;
+  std::memcpy(g.clip, v->clip_planes, sizeof(g.clip));
+  std::memcpy(g.key, v->keyhole_planes, sizeof(g.key));
+  g.zbuf = v->zbuffer;
+  g.zbw = v->zb_width;
+  g.zbh = v->zb_height;
+  return g;
+}
+
+int check_view(const rvk_view* v) {
+  if (!v) return fail(RVK_ERR_INVALID_ARGUMENT, "view is NULL");
+  if (v->width <= 0 || v->height <= 0 || v->width > 16384 || v->height > 16384)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "view size %dx%d out of range [1,16384]", v->width, v->height);
+  if (v->n_clip_planes < 0 || v->n_clip_planes > 6 || v->n_keyhole_planes < 0 || v->n_keyhole_planes > 6)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "plane counts must be within 0..6");
+  if (v->zbuffer && (v->zb_width <= 0 || v->zb_height <= 0))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "z-buffer given without a positive size");
+  return RVK_OK;
+}
+
+int check_frame(const rvk_volume* v, int frame) {
+  if (!v) return fail(RVK_ERR_INVALID_ARGUMENT, "volume is NULL");
+  if (frame < 0 || frame >= v->n_frames)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "frame %d out of range [0,%d)", frame, v->n_frames);
+  if (!v->frames[frame]) return fail(RVK_ERR_NOT_READY, "frame %d has no data", frame);
+  return RVK_OK;
+}
+
+int ensure_scratch(rvk_ctx* ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return RVK_OK;
+  if (ctx->scratch) {
+    RVK_HIP(hipStreamSynchronize(ctx->stream));
+    RVK_HIP(hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  RVK_HIP(hipMalloc(&ctx->scratch, bytes));
+  ctx->scratch_bytes = bytes;
+  return RVK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Linear (x fastest) -> 4x4x4 bricks.  One thread per bricked element, so the
+// stores are dense; padding voxels replicate the nearest edge voxel.
+template <typename VT>
+__global__ __launch_bounds__(256) void brick_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst,
+                                                             int nx, int ny, int nz, int nbx, int nby,
+                                                             long long total) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const long long b = e >> 6;
+  const int o = (int)(e & 63);
+  const int bx = (int)(b % nbx);
+  const int by = (int)((b / nbx) % nby);
+  const int bz = (int)(b / ((long long)nbx * nby));
+  int i = bx * 4 + (o & 3), j = by * 4 + ((o >> 2) & 3), k = bz * 4 + (o >> 4);
+  i = min(i, nx - 1);
+  j = min(j, ny - 1);
+  k = min(k, nz - 1);
+  dst[e] = src[(long long)i + (long long)j * nx + (long long)k * nx * ny];
+}
+
+// Gradient magnitude by central differences at voxel centres, the value the
+// reference's on-the-fly gradient takes there (tex3D at p +- 1 with clamp,
+// times 0.5/spacing; V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:263-271),
+// quantised to u8 as round(|g| / scale).
+template <typename VT>
+__global__ __launch_bounds__(256) void gradmag_kernel(const VT* __restrict__ vol, uint8_t* __restrict__ out,
+                                                      VolGeom g, float inv_scale, long long total) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const long long b = e >> 6;
+  const int o = (int)(e & 63);
+  const int nbx = g.bys >> 6, nby = (g.bzs / g.bys);
+  const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+  const int i = min(bx * 4 + (o & 3), g.n[0] - 1);
+  const int j = min(by * 4 + ((o >> 2) & 3), g.n[1] - 1);
+  const int k = min(bz * 4 + (o >> 4), g.n[2] - 1);
+  auto at = [&](int x, int y, int z) {
+    x = min(max(x, 0), g.n[0] - 1);
+    y = min(max(y, 0), g.n[1] - 1);
+    z = min(max(z, 0), g.n[2] - 1);
+    return (float)vol[brick_x(x) + brick_y(y, g.bys) + brick_z(z, g.bzs)];
+  };
+  const float gx = (at(i + 1, j, k) - at(i - 1, j, k)) * g.srecip[0];
+  const float gy = (at(i, j + 1, k) - at(i, j - 1, k)) * g.srecip[1];
+  const float gz = (at(i, j, k + 1) - at(i, j, k - 1)) * g.srecip[2];
+  const float q = sqrtf(gx * gx + gy * gy + gz * gz) * inv_scale;
+  out[e] = (uint8_t)fminf(rintf(q), 255.0f);
+}
+
+template <typename VT>
+static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
+  const long long total = (long long)v->padded_elems;
+  const int blocks = (int)((total + 255) / 256);
+  hipLaunchKernelGGL(brick_relayout_kernel<VT>, dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc,
+                     (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->nb[0], v->nb[1], total);
+  RVK_HIP(hipGetLastError());
+  return RVK_OK;
+}
+
+template <typename VT>
+static int launch_gradmag(rvk_volume* v, int frame, float inv_scale) {
+  const long long total = (long long)v->padded_elems;
+  const int blocks = (int)((total + 255) / 256);
+  hipLaunchKernelGGL(gradmag_kernel<VT>, dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)v->frames[frame],
+                     v->gradmag[frame], vol_geom(v), inv_scale, total);
+  RVK_HIP(hipGetLastError());
+  return RVK_OK;
+}
+
+template <typename F> static int dispatch_dtype(int dtype, F&& f) {
+  switch (dtype) {
+    case RVK_U8: return f((uint8_t*)nullptr);
+    case RVK_I8: return f((int8_t*)nullptr);
+    case RVK_U16: return f((uint16_t*)nullptr);
+    case RVK_I16: return f((int16_t*)nullptr);
+    case RVK_U32: return f((uint32_t*)nullptr);
+    case RVK_I32: return f((int32_t*)nullptr);
+    case RVK_F32: return f((float*)nullptr);
+    default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+  }
+}
+
+}  // namespace rvk
+
+using namespace rvk;
+
+extern "C" {
+
+const char* rvk_last_error_string(void) { return g_last_error.c_str(); }
+
+const char* rvk_status_string(int status) {
+  switch (status) {
+    case RVK_OK: return "ok";
+    case RVK_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case RVK_ERR_NO_DEVICE: return "no device";
+    case RVK_ERR_HIP: return "HIP runtime error";
+    case RVK_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case RVK_ERR_NOT_READY: return "not ready";
+    case RVK_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+int rvk_version(void) { return RVK_VERSION; }
+
+int rvk_device_count(int* count) {
+  if (!count) return fail(RVK_ERR_INVALID_ARGUMENT, "count is NULL");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e == hipErrorNoDevice) { *count = 0; return RVK_OK; }
+  RVK_HIP(e);
+  return RVK_OK;
+}
+
+static int ctx_create(int device, bool borrow, void* stream, rvk_ctx** out) {
+  if (!out) return fail(RVK_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(RVK_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(RVK_ERR_NO_DEVICE, "device %d out of range [0,%d)", device, n);
+  RVK_HIP(hipSetDevice(device));
+  rvk_ctx* c = new rvk_ctx();
+  c->device = device;
+  if (borrow) {
+    c->stream = (hipStream_t)stream;  // NULL is the device's default stream
+  } else {
+    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return hip_fail(e, "hipStreamCreateWithFlags"); }
+    c->stream = c->own_stream;
+  }
+  *out = c;
+  return RVK_OK;
+}
+
+int rvk_ctx_create(int device, rvk_ctx** out) { return ctx_create(device, false, nullptr, out); }
+int rvk_ctx_create_on_stream(int device, void* s, rvk_ctx** out) { return ctx_create(device, true, s, out); }
+
+int rvk_ctx_destroy(rvk_ctx* ctx) {
+  if (!ctx) return RVK_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return RVK_OK;
+}
+
+int rvk_ctx_device(const rvk_ctx* ctx, int* device) {
+  if (!ctx || !device) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *device = ctx->device;
+  return RVK_OK;
+}
+
+int rvk_ctx_stream(const rvk_ctx* ctx, void** s) {
+  if (!ctx || !s) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *s = (void*)ctx->stream;
+  return RVK_OK;
+}
+
+int rvk_ctx_set_stream(rvk_ctx* ctx, void* s) {
+  if (!ctx) return fail(RVK_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  if (!s && !ctx->own_stream) {
+    RVK_HIP(hipSetDevice(ctx->device));
+    RVK_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+  }
+  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  return RVK_OK;
+}
+
+int rvk_ctx_sync(rvk_ctx* ctx) {
+  if (!ctx) return fail(RVK_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  RVK_HIP(hipSetDevice(ctx->device));
+  RVK_HIP(hipStreamSynchronize(ctx->stream));
+  return RVK_OK;
+}
+
+// ---- volumes ----------------------------------------------------------------
+int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[3], int dtype, int n_frames,
+                      rvk_volume** out) {
+  if (!ctx || !dims || !spacing || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (dtype_size(dtype) == 0) return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+  if (n_frames < 1 || n_frames > RVK_MAX_FRAMES)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "n_frames %d out of range [1,%d]", n_frames, RVK_MAX_FRAMES);
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] < 2 || dims[a] > 4096) return fail(RVK_ERR_INVALID_ARGUMENT, "dims[%d]=%d out of range [2,4096]", a, dims[a]);
+    if (!(spacing[a] > 0.0) || !std::isfinite(spacing[a]))
+      return fail(RVK_ERR_INVALID_ARGUMENT, "spacing[%d] must be positive and finite", a);
+  }
+  rvk_volume* v = new rvk_volume();
+  v->ctx = ctx;
+  v->dtype = dtype;
+  v->n_frames = n_frames;
+  for (int a = 0; a < 3; ++a) {
+    v->dims[a] = dims[a];
+    v->spacing[a] = spacing[a];
+    v->nb[a] = (dims[a] + 3) / 4;
+  }
+  v->padded_elems = (size_t)64 * v->nb[0] * v->nb[1] * v->nb[2];
+  if (v->padded_elems > (size_t)0x7fffffff) {
+    delete v;
+    return fail(RVK_ERR_INVALID_ARGUMENT, "volume too large for 32-bit element offsets");
+  }
+  *out = v;
+  return RVK_OK;
+}
+
+int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_device) {
+  if (!v || !src) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (frame < 0 || frame >= v->n_frames)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "frame %d out of range [0,%d)", frame, v->n_frames);
+  RVK_HIP(hipSetDevice(v->ctx->device));
+  const size_t es = dtype_size(v->dtype);
+  const size_t n = (size_t)v->dims[0] * v->dims[1] * v->dims[2];
+  if (!v->frames[frame]) RVK_HIP(hipMalloc(&v->frames[frame], v->padded_elems * es));
+  const void* dsrc = src;
+  void* staging = nullptr;
+  if (!src_on_device) {
+    RVK_HIP(hipMalloc(&staging, n * es));
+    hipError_t e = hipMemcpyAsync(staging, src, n * es, hipMemcpyHostToDevice, v->ctx->stream);
+    if (e != hipSuccess) { hipFree(staging); return hip_fail(e, "hipMemcpyAsync(volume)"); }
+    dsrc = staging;
+  }
+  int rc = dispatch_dtype(v->dtype, [&](auto tag) {
+    using VT = std::remove_pointer_t<decltype(tag)>;
+    return launch_relayout<VT>(v, dsrc, v->frames[frame]);
+  });
+  if (staging) {
+    hipError_t e = hipStreamSynchronize(v->ctx->stream);
+    hipFree(staging);
+    if (rc == RVK_OK && e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  }
+  return rc;
+}
+
+int rvk_volume_compute_gradmag(rvk_volume* v, int frame, float max_gradient) {
+  int rc = check_frame(v, frame);
+  if (rc) return rc;
+  if (!(max_gradient > 0.0f) || !std::isfinite(max_gradient))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "max_gradient must be positive and finite");
+  RVK_HIP(hipSetDevice(v->ctx->device));
+  if (!v->gradmag[frame]) RVK_HIP(hipMalloc((void**)&v->gradmag[frame], v->padded_elems));
+  v->gradmag_scale[frame] = max_gradient / 255.0f;
+  const float inv = 255.0f / max_gradient;
+  return dispatch_dtype(v->dtype, [&](auto tag) {
+    using VT = std::remove_pointer_t<decltype(tag)>;
+    return launch_gradmag<VT>(v, frame, inv);
+  });
+}
+
+int rvk_volume_destroy(rvk_volume* v) {
+  if (!v) return RVK_OK;
+  hipSetDevice(v->ctx->device);
+  hipStreamSynchronize(v->ctx->stream);
+  for (int f = 0; f < RVK_MAX_FRAMES; ++f) {
+    if (v->frames[f]) hipFree(v->frames[f]);
+    if (v->gradmag[f]) hipFree(v->gradmag[f]);
+  }
+  delete v;
+  return RVK_OK;
+}
+
+int rvk_volume_dims(const rvk_volume* v, int32_t dims[3], double spacing[3], int* dtype, int* n_frames) {
+  if (!v) return fail(RVK_ERR_INVALID_ARGUMENT, "volume is NULL");
+  for (int a = 0; a < 3; ++a) {
+    if (dims) dims[a] = v->dims[a];
+    if (spacing) spacing[a] = v->spacing[a];
+  }
+  if (dtype) *dtype = v->dtype;
+  if (n_frames) *n_frames = v->n_frames;
+  return RVK_OK;
+}
+
+// ---- transfer functions -------------------------------------------------------
+int rvk_tf1d_create(rvk_ctx* ctx, int n, const float* alpha, const float* galpha, const float* red,
+                    const float* green, const float* blue, const rvk_tf1d_params* p, rvk_tf1d** out) {
+  if (!ctx || !alpha || !red || !green || !blue || !p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (n < 1 || n > 4096) return fail(RVK_ERR_INVALID_ARGUMENT, "TF size %d out of range [1,4096]", n);
+  std::vector<float> h((size_t)5 * n);
+  std::memcpy(&h[0], alpha, sizeof(float) * n);
+  for (int i = 0; i < n; ++i) h[(size_t)n + i] = galpha ? galpha[i] : 1.0f;
+  std::memcpy(&h[(size_t)2 * n], red, sizeof(float) * n);
+  std::memcpy(&h[(size_t)3 * n], green, sizeof(float) * n);
+  std::memcpy(&h[(size_t)4 * n], blue, sizeof(float) * n);
+  RVK_HIP(hipSetDevice(ctx->device));
+  rvk_tf1d* t = new rvk_tf1d();
+  t->ctx = ctx;
+  t->n = n;
+  t->p = *p;
+  hipError_t e = hipMalloc((void**)&t->tables, h.size() * sizeof(float));
+  if (e != hipSuccess) { delete t; return hip_fail(e, "hipMalloc(tf1d)"); }
+  e = hipMemcpy(t->tables, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { hipFree(t->tables); delete t; return hip_fail(e, "hipMemcpy(tf1d)"); }
+  *out = t;
+  return RVK_OK;
+}
+
+int rvk_tf1d_destroy(rvk_tf1d* t) {
+  if (!t) return RVK_OK;
+  hipSetDevice(t->ctx->device);
+  hipStreamSynchronize(t->ctx->stream);
+  if (t->tables) hipFree(t->tables);
+  delete t;
+  return RVK_OK;
+}
+
+static int upload_cells(int n, const float* const tables[8], Tf2dCell** dst) {
+  for (int i = 0; i < 8; ++i)
+    if (!tables[i]) return fail(RVK_ERR_INVALID_ARGUMENT, "2D TF table %d is NULL", i);
+  const size_t nn = (size_t)n * n;
+  std::vector<Tf2dCell> h(nn);
+  for (size_t c = 0; c < nn; ++c) {
+    h[c].alpha = tables[0][c];
+    h[c].r = tables[1][c];
+    h[c].g = tables[2][c];
+    h[c].b = tables[3][c];
+    h[c].ambient = tables[4][c];
+    h[c].diffuse = tables[5][c];
+    h[c].specular = tables[6][c];
+    h[c].spec_power = tables[7][c];
+  }
+  RVK_HIP(hipMalloc((void**)dst, nn * sizeof(Tf2dCell)));
+  hipError_t e = hipMemcpy(*dst, h.data(), nn * sizeof(Tf2dCell), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { hipFree(*dst); *dst = nullptr; return hip_fail(e, "hipMemcpy(tf2d)"); }
+  return RVK_OK;
+}
+
+int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const float* const ktables[8],
+                    const rvk_tf2d_params* p, rvk_tf2d** out) {
+  if (!ctx || !tables || !p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (n < 1 || n > 2048) return fail(RVK_ERR_INVALID_ARGUMENT, "TF size %d out of range [1,2048]", n);
+  RVK_HIP(hipSetDevice(ctx->device));
+  rvk_tf2d* t = new rvk_tf2d();
+  t->ctx = ctx;
+  t->n = n;
+  t->p = *p;
+  int rc = upload_cells(n, tables, &t->cells);
+  if (rc == RVK_OK && ktables) rc = upload_cells(n, ktables, &t->kcells);
+  if (rc != RVK_OK) {
+    if (t->cells) hipFree(t->cells);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return RVK_OK;
+}
+
+int rvk_tf2d_destroy(rvk_tf2d* t) {
+  if (!t) return RVK_OK;
+  hipSetDevice(t->ctx->device);
+  hipStreamSynchronize(t->ctx->stream);
+  if (t->cells) hipFree(t->cells);
+  if (t->kcells) hipFree(t->kcells);
+  delete t;
+  return RVK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,312 @@
+"""Parity of the gfx950 casters (called through the C-ABI) with the CPU oracle.
+
+Tolerances: DRR float outputs <= 1e-4 relative error against the FP64 oracle
+(replaying the FP32 branch decisions) and, against the FP32 oracle in the
+same march mode, <= 2e-6 (both follow the same operation order; only
+expf/logf differ); RGBA8 outputs within 1 LSB on >= 99.9 % of pixels.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from robartsvtk_amd import abi, configs
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+DRR_REL_TOL = 1e-4
+SAME_MODE_REL_TOL = 2e-6
+
+
+def _drr_check(ctx, vol, spacing, view, march, drr, tol_f64=DRR_REL_TOL, tol_f32=SAME_MODE_REL_TOL):
+    vh = H.gpu_volume(ctx, vol, spacing)
+    g = H.gpu_drr(ctx, vh, view, march, drr)
+    vd = configs.view_dict(view)
+    o32 = orc.drr(vol, spacing, vd, march, drr, "f32")
+    o64 = orc.drr(vol, spacing, vd, dict(march, f32_control=1), drr, "f64")
+    vh.close()
+    if drr["output"] == abi.DRR_RGBA8_REF:
+        frac32, mx32 = H.lsb_stats(g, o32)
+        frac64, mx64 = H.lsb_stats(g, o64)
+        print(f"rgba8 within 1 LSB: vs f32 {frac32:.6f} (max {mx32}), vs f64 {frac64:.6f} (max {mx64})")
+        assert frac32 >= 0.999 and frac64 >= 0.999
+        return g
+    assert np.all(np.isfinite(g))
+    e32 = H.rel_err(g, o32).max()
+    e64 = H.rel_err(g, o64).max()
+    print(f"max rel err vs f32 oracle {e32:.3e}, vs f64 oracle {e64:.3e}")
+    assert e32 <= tol_f32, e32
+    assert e64 <= tol_f64, e64
+    return g
+
+
+# ---------------------------------------------------------------------------
+# DRR
+@pytest.mark.parametrize("dtype", ["u8", "i8", "u16", "i16", "u32", "i32", "f32"])
+def test_drr_all_voxel_types(gpu_ctx, dtype):
+    s = configs.scene_c2(scale=0.125)
+    hu = s.volume.astype(np.float64)
+    drr = dict(s.drr)
+    # store the phantom in each native type, with slope/intercept giving the same mu
+    if dtype == "u8":
+        vol = np.clip(np.rint((hu + 1000.0) / 10.0), 0, 255).astype(np.uint8)
+        drr.update(slope=2e-4, intercept=0.0)
+    elif dtype == "i8":
+        vol = np.clip(np.rint(hu / 20.0), -128, 127).astype(np.int8)
+        drr.update(slope=4e-4, intercept=0.02)
+    elif dtype == "u16":
+        vol = (hu + 1024.0).astype(np.uint16)
+        drr.update(intercept=0.02 - 1024 * 2e-5)
+    elif dtype == "i16":
+        vol = s.volume
+    elif dtype == "u32":
+        vol = (hu + 1024.0).astype(np.uint32)
+        drr.update(intercept=0.02 - 1024 * 2e-5)
+    elif dtype == "i32":
+        vol = hu.astype(np.int32)
+    else:
+        vol = configs.mu_from_hu(s.volume)
+        drr.update(slope=1.0, intercept=0.0)
+    _drr_check(gpu_ctx, vol, s.spacing, s.views[0], s.march, drr)
+
+
+@pytest.mark.parametrize("skip", [0, 1])
+@pytest.mark.parametrize("precise", [0, 1])
+@pytest.mark.parametrize("jitter", [0, 1])
+def test_drr_march_modes(gpu_ctx, skip, precise, jitter):
+    s = configs.scene_c2(scale=0.25)
+    march = dict(s.march, skip_rule=skip, precise=precise, jitter=jitter)
+    # the reference (non-precise) FP32 march drifts; its FP64 gap is only reported
+    tol64 = DRR_REL_TOL if precise else 1.0
+    _drr_check(gpu_ctx, s.volume, s.spacing, s.views[0], march, s.drr, tol_f64=tol64)
+
+
+@pytest.mark.parametrize("output", [abi.DRR_RGBA8_REF, abi.DRR_F32_EXP_NEG_L, abi.DRR_F32_L])
+@pytest.mark.parametrize("ref_term", [0, 1])
+def test_drr_outputs_and_termination(gpu_ctx, output, ref_term):
+    s = configs.scene_c2(scale=0.25)
+    drr = dict(s.drr, output=output, ref_termination=ref_term, tint=(200, 120, 40, 90), offset=0.9)
+    _drr_check(gpu_ctx, s.volume, s.spacing, s.views[0], s.march, drr)
+
+
+def test_drr_reference_defaults(gpu_ctx):
+    """Reference constants: CT 0.022/0.000022/1.0, step = MinSpacing, RGBA8."""
+    s = configs.scene_c2(scale=0.25)
+    march = {"step_scale": 1.0, "term_T": 0.0, "jitter": 1, "skip_rule": 1, "precise": 0}
+    drr = {"intercept": 0.022, "slope": 0.000022, "offset": 1.0, "output": abi.DRR_RGBA8_REF,
+           "ref_termination": 1, "tint": (0, 0, 0, 0)}
+    _drr_check(gpu_ctx, s.volume, s.spacing, s.views[0], march, drr)
+
+
+def test_drr_c2_full_size(gpu_ctx):
+    """The benchmarked DRR workload at full size (256^3 i16 -> 1024^2 f32)."""
+    s = configs.scene_c2(scale=1.0)
+    g = _drr_check(gpu_ctx, s.volume, s.spacing, s.views[0], s.march, s.drr)
+    assert g.min() > 0.0 and g.max() <= 1.0
+
+
+# ---------------------------------------------------------------------------
+# geometry edge cases
+def _small_drr_scene():
+    s = configs.scene_c2(scale=0.125)
+    return s
+
+
+@pytest.mark.parametrize("wh", [(1, 1), (37, 23), (16, 16), (130, 7)])
+def test_drr_ragged_sizes(gpu_ctx, wh):
+    s = _small_drr_scene()
+    cam = s.extra["camera"]
+    view = abi.compute_view(cam, wh[0], wh[1], spacing=s.spacing)
+    _drr_check(gpu_ctx, s.volume, s.spacing, view, s.march, s.drr)
+
+
+def test_drr_rays_missing_volume(gpu_ctx):
+    s = _small_drr_scene()
+    c = configs.volume_center(s.dims, s.spacing)
+    cam = abi.camera(c + np.array([600.0, 0, 0]), c + np.array([1200.0, 0, 0]), (0, 0, 1), 20.0, (10.0, 400.0))
+    view = abi.compute_view(cam, 40, 24, spacing=s.spacing)
+    g = _drr_check(gpu_ctx, s.volume, s.spacing, view, s.march, s.drr)
+    assert np.all(g == 1.0)  # exp(-0)
+
+
+def test_drr_flipped_clip_and_keyhole(gpu_ctx):
+    s = configs.scene_c2(scale=0.25)
+    view = abi.compute_view(s.extra["camera"], 96, 80, spacing=s.spacing, flipped=True)
+    n = s.dims[0]
+    planes = [(1, 0, 0, -0.2 * n), (-1, 0, 0, 0.9 * n), (0, 1, 0, -0.1 * n), (0, -1, 0, 0.8 * n),
+              (0, 0, 1, -0.15 * n), (0, 0, -1, 0.95 * n)]
+    view.n_clip_planes = 6
+    view.clip_planes[:] = [float(x) for p in planes for x in p]
+    key = [(0, 0, 1, -0.5 * n), (0, 0, -1, 0.7 * n), (1, 0, 0, -0.3 * n), (-1, 0, 0, 0.6 * n),
+           (0, 1, 0, -0.3 * n), (0, -1, 0, 0.7 * n)]
+    view.n_keyhole_planes = 6
+    view.keyhole_planes[:] = [float(x) for p in key for x in p]
+    for skip in (0, 1):
+        _drr_check(gpu_ctx, s.volume, s.spacing, view, dict(s.march, skip_rule=skip), s.drr)
+
+
+def test_drr_zbuffer(gpu_ctx):
+    import torch
+    s = _small_drr_scene()
+    view = abi.compute_view(s.extra["camera"], 64, 48, spacing=s.spacing)
+    zb = np.linspace(0.3, 1.0, 50 * 30, dtype=np.float32).reshape(30, 50)
+    zb_dev = torch.from_numpy(zb).cuda()
+    view.zbuffer = zb_dev.data_ptr()
+    view.zb_width, view.zb_height = 50, 30
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g = H.gpu_drr(gpu_ctx, vh, view, s.march, s.drr)
+    vd = dict(configs.view_dict(view), zbuffer=zb)
+    o32 = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f32")
+    assert H.rel_err(g, o32).max() <= SAME_MODE_REL_TOL
+
+
+# ---------------------------------------------------------------------------
+# 1D transfer function
+def _check_rgba(g, o32, o64, label):
+    f32, m32 = H.lsb_stats(g, o32)
+    f64, m64 = H.lsb_stats(g, o64)
+    exact = float(np.mean(np.all(g == o32, axis=-1)))
+    print(f"{label}: within 1 LSB vs f32 {f32:.6f} (max {m32}, bit-exact {exact:.6f}); vs f64 {f64:.6f} (max {m64})")
+    assert f32 >= 0.999 and f64 >= 0.999
+
+
+@pytest.mark.parametrize("scale", [0.25, 1.0])
+def test_1d_c1(gpu_ctx, scale):
+    s = configs.scene_c1(scale=scale)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g = H.gpu_1d(gpu_ctx, vh, s.tf1d, s.views[0], s.march)
+    vd = configs.view_dict(s.views[0])
+    o32, _, _ = orc.cast1d(s.volume, s.spacing, s.tf1d, vd, s.march, "f32")
+    o64, _, _ = orc.cast1d(s.volume, s.spacing, s.tf1d, vd, dict(s.march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, f"c1 scale {scale}")
+    assert g[..., 3].max() > 200  # something opaque was hit
+
+
+@pytest.mark.parametrize("skip", [0, 1])
+@pytest.mark.parametrize("precise", [0, 1])
+def test_1d_shaded_gradient_opacity(gpu_ctx, skip, precise):
+    s = configs.scene_c1(scale=0.25)
+    tf = dict(s.tf1d)
+    n = tf["alpha"].shape[0]
+    tf["galpha"] = np.linspace(0.2, 1.0, n).astype(np.float32)
+    tf["params"] = dict(tf["params"], gradient_low=0.0, gradient_multiplier=1.0 / 40.0, ambient=0.3, diffuse=0.6,
+                        specular=0.4, specular_power=8.0)
+    march = dict(s.march, skip_rule=skip, precise=precise, term_T=1.0 / 64)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g = H.gpu_1d(gpu_ctx, vh, tf, s.views[0], march)
+    vd = configs.view_dict(s.views[0])
+    o32, _, _ = orc.cast1d(s.volume, s.spacing, tf, vd, march, "f32")
+    o64, _, _ = orc.cast1d(s.volume, s.spacing, tf, vd, dict(march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, "1d shaded")
+
+
+def test_1d_aux_and_post_shade(gpu_ctx):
+    import torch
+    s = configs.scene_c1(scale=0.25)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g, (depth, mind, maxd) = H.gpu_1d(gpu_ctx, vh, s.tf1d, s.views[0], s.march, with_aux=True)
+    vd = configs.view_dict(s.views[0])
+    o32, _, odepth = orc.cast1d(s.volume, s.spacing, s.tf1d, vd, s.march, "f32")
+    rays = orc.form_rays(s.dims, s.spacing, vd, s.march, "f32")
+    assert np.array_equal(depth, odepth)
+    assert np.array_equal(mind, rays["min_depth"]) and np.array_equal(maxd, rays["max_depth"])
+    shade = {"celr": 0.7, "cela": 0.002, "celc": 1.0 / (0.005 - 0.002), "disr": 0.25, "disa": 0.45,
+             "disc": 1.0 / (0.6 - 0.45)}
+    rgba = torch.from_numpy(np.ascontiguousarray(g)).cuda()
+    dd, dmin, dmax = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (depth, mind, maxd))
+    sp = abi.ShadeParams(*[shade[k] for k in ("celr", "cela", "celc", "disr", "disa", "disc")])
+    abi.check(abi.lib().rvk_post_shade(gpu_ctx.handle, g.shape[1], g.shape[0], sp, dd.data_ptr(), dmin.data_ptr(),
+                                       dmax.data_ptr(), rgba.data_ptr()))
+    torch.cuda.synchronize()
+    od, orgba = orc.post_shade(shade, depth, mind, maxd, g)
+    assert np.array_equal(rgba.cpu().numpy(), orgba)
+    assert np.array_equal(dd.cpu().numpy(), od)
+
+
+# ---------------------------------------------------------------------------
+# 2D transfer function
+def _scene_c3_small():
+    return configs.scene_c3(scale=0.25)
+
+
+def test_2d_c3_gradmag(gpu_ctx):
+    s = _scene_c3_small()
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    gmax = s.tf2d["gradmag_max"]
+    vh.compute_gradmag(0, gmax)
+    g = H.gpu_2d(gpu_ctx, vh, s.tf2d, s.views[0], s.march, use_gradmag=True)
+    gm = H.gradmag_u8(s.volume, s.spacing, gmax)
+    tf = dict(s.tf2d, gradmag=gm, gradmag_scale=float(np.float32(gmax) / np.float32(255.0)))
+    vd = configs.view_dict(s.views[0])
+    o32, _, _ = orc.cast2d(s.volume, s.spacing, tf, vd, s.march, "f32")
+    o64, _, _ = orc.cast2d(s.volume, s.spacing, tf, vd, dict(s.march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, "c3 gradmag")
+    assert g[..., 3].max() > 100
+
+
+@pytest.mark.parametrize("skip", [0, 1])
+def test_2d_on_the_fly_gradient_shaded(gpu_ctx, skip):
+    s = _scene_c3_small()
+    tf = dict(s.tf2d)
+    N = tf["tables"]["alpha"].shape[0]
+    r = np.random.default_rng(7)
+    tables = dict(tf["tables"])
+    tables["ambient"] = (0.2 + 0.3 * r.random((N, N))).astype(np.float32)
+    tables["diffuse"] = (0.5 * r.random((N, N))).astype(np.float32)
+    tables["specular"] = (0.4 * r.random((N, N))).astype(np.float32)
+    tables["specular_power"] = (1.0 + 10.0 * r.random((N, N))).astype(np.float32)
+    tf["tables"] = tables
+    march = dict(s.march, skip_rule=skip, term_T=1.0 / 32)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g = H.gpu_2d(gpu_ctx, vh, tf, s.views[0], march, use_gradmag=False)
+    vd = configs.view_dict(s.views[0])
+    o32, _, _ = orc.cast2d(s.volume, s.spacing, tf, vd, march, "f32")
+    o64, _, _ = orc.cast2d(s.volume, s.spacing, tf, vd, dict(march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, "2d shaded")
+
+
+def test_2d_keyhole_dual_tf(gpu_ctx):
+    s = _scene_c3_small()
+    tf = dict(s.tf2d)
+    tf["keyhole_tables"] = configs.tf2d_bumps(99, tf["tables"]["alpha"].shape[0])
+    view = abi.compute_view(s.extra["camera"], 120, 90, spacing=s.spacing)
+    n = s.dims[0]
+    key = [(1, 0, 0, -0.3 * n), (-1, 0, 0, 0.7 * n), (0, 1, 0, -0.3 * n), (0, -1, 0, 0.7 * n),
+           (0, 0, 1, -0.2 * n), (0, 0, -1, 0.8 * n)]
+    view.n_keyhole_planes = 6
+    view.keyhole_planes[:] = [float(x) for p in key for x in p]
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g, (depth, _, _) = H.gpu_2d(gpu_ctx, vh, tf, view, s.march, use_gradmag=False, with_aux=True)
+    vd = configs.view_dict(view)
+    o32, _, od = orc.cast2d(s.volume, s.spacing, tf, vd, s.march, "f32")
+    o64, _, _ = orc.cast2d(s.volume, s.spacing, tf, vd, dict(s.march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, "2d dual")
+
+
+# ---------------------------------------------------------------------------
+# error behaviour through the ABI
+def test_errors(gpu_ctx):
+    import torch
+    v = abi.Volume(gpu_ctx, (8, 8, 8), (1, 1, 1), "u8", n_frames=2)
+    cam = abi.camera((3.5, 3.5, 40.0), (3.5, 3.5, 3.5), (0, 1, 0), 20.0, (10.0, 80.0))
+    view = abi.compute_view(cam, 4, 4)
+    out = torch.zeros(16, dtype=torch.float32, device="cuda")
+    m = abi.MarchParams(1.0, 0.0, 1, 1, 1)
+    d = configs.drr_params({"intercept": 0.0, "slope": 2e-5, "offset": 1.0, "output": 1, "ref_termination": 0})
+    with pytest.raises(abi.RvkError) as e:
+        abi.render_drr(gpu_ctx, v, view, m, d, out, frame=0)
+    assert e.value.status == 5  # not ready: frame has no data
+    with pytest.raises(abi.RvkError) as e:
+        abi.render_drr(gpu_ctx, v, view, m, d, out, frame=7)
+    assert e.value.status == 1
+    v.set_frame(1, np.zeros((8, 8, 8), np.uint8))
+    abi.render_drr(gpu_ctx, v, view, m, d, out, frame=1)
+    torch.cuda.synchronize()
+    assert torch.all(out == 1.0)
+    with pytest.raises(abi.RvkError):
+        abi.Volume(gpu_ctx, (8, 8, 1), (1, 1, 1), "u8")
+    with pytest.raises(abi.RvkError):
+        abi.render_drr(gpu_ctx, v, view, abi.MarchParams(0.0, 0.0, 1, 1, 1), d, out, frame=1)

@@ -172,7 +172,7 @@ int rvk_volume_dims(const rvk_volume* vol, int32_t dims[3], double spacing[3], i
 /* ---- transfer functions -----------------------------------------------------
  * 1D: replaces CUDA_vtkCuda1DVolumeMapper_renderAlgo_loadTextures
  * (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.h:61-64).  Tables of n floats
- * (n <= 4096); galpha may be NULL (all ones). */
+ * (n <= 2048; the five tables are staged in LDS); galpha may be NULL (all ones). */
 int rvk_tf1d_create(rvk_ctx* ctx, int n, const float* alpha, const float* galpha, const float* red,
                     const float* green, const float* blue, const rvk_tf1d_params* params, rvk_tf1d** out);
 int rvk_tf1d_destroy(rvk_tf1d* tf);

@@ -394,7 +394,7 @@ int rvk_tf1d_create(rvk_ctx* ctx, int n, const float* alpha, const float* galpha
                     const float* green, const float* blue, const rvk_tf1d_params* p, rvk_tf1d** out) {
   if (!ctx || !alpha || !red || !green || !blue || !p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = nullptr;
-  if (n < 1 || n > 4096) return fail(RVK_ERR_INVALID_ARGUMENT, "TF size %d out of range [1,4096]", n);
+  if (n < 1 || n > 2048) return fail(RVK_ERR_INVALID_ARGUMENT, "TF size %d out of range [1,2048]", n);
   std::vector<float> h((size_t)5 * n);
   std::memcpy(&h[0], alpha, sizeof(float) * n);
   for (int i = 0; i < n; ++i) h[(size_t)n + i] = galpha ? galpha[i] : 1.0f;

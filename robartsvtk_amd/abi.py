@@ -127,6 +127,14 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RvkError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build() (or robartsvtk_amd/build.py)")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7.
+        # Loading it first makes this library bind to the same copy (same
+        # soname); the other order leaves torch on a second runtime that
+        # cannot see the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGS.items():
             fn = getattr(_lib, name, None)

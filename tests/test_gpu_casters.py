@@ -35,8 +35,13 @@ def _drr_check(ctx, vol, spacing, view, march, drr, tol_f64=DRR_REL_TOL, tol_f32
         assert frac32 >= 0.999 and frac64 >= 0.999
         return g
     assert np.all(np.isfinite(g))
-    e32 = H.rel_err(g, o32).max()
-    e64 = H.rel_err(g, o64).max()
+    if drr["output"] == abi.DRR_F32_L:
+        # intensity exp(-L) has relative error |dL|: compare L absolutely
+        e32 = np.abs(g - o32).max()
+        e64 = np.abs(g - o64).max()
+    else:
+        e32 = H.rel_err(g, o32).max()
+        e64 = H.rel_err(g, o64).max()
     print(f"max rel err vs f32 oracle {e32:.3e}, vs f64 oracle {e64:.3e}")
     assert e32 <= tol_f32, e32
     assert e64 <= tol_f64, e64

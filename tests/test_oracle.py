@@ -1,0 +1,385 @@
+"""Pins of the CPU oracle (CPU only).
+
+Each test checks the oracle against something other than its own formula:
+the reference's jitter table (golden fixture), closed forms (linear fields,
+uniform media, parallel-beam chord lengths, Beer-Lambert), symmetries, and a
+second, naive restatement of the DRR march written separately below.
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from robartsvtk_amd import abi, configs
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_jitter_table_matches_reference_fixture():
+    ref = json.loads((GOLDEN / "ray_offsets.json").read_text())
+    vals = np.array([np.float32(float(v)) for v in ref["values"]], np.float32)
+    assert vals.shape == (256,)
+    np.testing.assert_array_equal(orc.jitter_table(), vals)
+
+
+# ---------------------------------------------------------------------------
+# sampling
+def test_trilinear_reproduces_linear_fields_and_clamps():
+    nx, ny, nz = 7, 9, 11
+    a, b, c, d = 3.0, 2.0, -5.0, 7.0
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    vol = (a + b * i + c * j + d * k).astype(np.float32)
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        p = rng.uniform(-2.0, 13.0, 3)
+        x = min(max(p[0] - 0.5, 0.0), nx - 1)
+        y = min(max(p[1] - 0.5, 0.0), ny - 1)
+        z = min(max(p[2] - 0.5, 0.0), nz - 1)
+        expect = a + b * x + c * y + d * z
+        assert abs(orc.sample(vol, (1, 1, 1), p, "f64") - expect) < 1e-9
+        assert abs(orc.sample(vol, (1, 1, 1), p, "f32") - expect) < 1e-3
+
+
+def test_tf_lookups_linear_and_point():
+    n = 16
+    t = np.arange(n, dtype=np.float32)
+    for u in np.linspace(-0.3, 1.3, 97):
+        assert orc.tf1d_lookup(t, u) == pytest.approx(min(max(u * n - 0.5, 0.0), n - 1), abs=1e-5)
+    t2 = (np.arange(n)[:, None] * 1000 + np.arange(n)[None, :]).astype(np.float32)  # [g, i]
+    for u, v in [(0.0, 0.0), (0.99, 0.01), (0.5, 0.26), (-1.0, 2.0), (0.0624, 0.0626)]:
+        i = min(max(int(math.floor(u * n)), 0), n - 1)
+        g = min(max(int(math.floor(v * n)), 0), n - 1)
+        assert orc.tf2d_lookup(t2, u, v) == g * 1000 + i
+
+
+# ---------------------------------------------------------------------------
+# geometry
+def _parallel_z_view(dims, spacing, W, H, scale=None):
+    ext = [dims[a] * spacing[a] for a in range(3)]
+    c = configs.volume_center(dims, spacing)
+    S = scale or 0.5 * max(ext[0], ext[1]) * 1.2
+    cam = abi.camera(c + np.array([0, 0, 3.0 * ext[2]]), c, (0, 1, 0), 30.0, (ext[2], 5.0 * ext[2]), parallel=True,
+                     parallel_scale=S)
+    return abi.compute_view(cam, W, H, spacing=spacing), c, S
+
+
+def test_ray_setup_parallel_projection_closed_form():
+    dims, spacing = (20, 24, 28), (1.0, 1.5, 2.0)
+    W, H = 32, 24
+    view, c, S = _parallel_z_view(dims, spacing, W, H)
+    march = {"step_scale": 0.5, "jitter": 0, "skip_rule": 0}
+    r = orc.form_rays(dims, spacing, configs.view_dict(view), march, "f64")
+    aspect = W / H
+    for y in (3, 12, 20):
+        for x in (5, 16, 27):
+            wx = c[0] + (2.0 * x / W - 1.0) * S * aspect
+            wy = c[1] + (2.0 * y / H - 1.0) * S
+            vx, vy = wx / spacing[0], wy / spacing[1]
+            inside = 1.0 <= vx <= dims[0] - 2 and 1.0 <= vy <= dims[1] - 2
+            st, inc, ns = r["start"][y, x], r["inc"][y, x], r["num_steps"][y, x]
+            if not inside:
+                continue
+            # the view matrix is handed over in float32, as the reference stores it
+            assert st[0] == pytest.approx(vx, abs=1e-4) and st[1] == pytest.approx(vy, abs=1e-4)
+            assert st[2] == pytest.approx(dims[2] - 2, abs=1e-9)  # enters through the clipped top face
+            assert abs(inc[0]) < 1e-5 and abs(inc[1]) < 1e-5 and inc[2] < 0
+            length_mm = (dims[2] - 3) * spacing[2]
+            assert ns == pytest.approx(length_mm / (min(spacing) * 0.5), rel=1e-6)
+
+
+def test_ray_setup_perspective_center_ray_hits_focal_point():
+    s = configs.scene_c1(scale=0.25)
+    view = abi.compute_view(s.extra["camera"], 64, 64, spacing=s.spacing)
+    r = orc.form_rays(s.dims, s.spacing, configs.view_dict(view), s.march, "f64")
+    st, inc = r["start"][32, 32], r["inc"][32, 32]
+    f = np.asarray(s.extra["camera"].focal_point[:]) / np.asarray(s.spacing)
+    cr = np.cross(inc, f - st)
+    assert np.linalg.norm(cr) / (np.linalg.norm(inc) * np.linalg.norm(f - st)) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# DRR closed forms
+def test_drr_uniform_box_parallel_counts_and_jitter():
+    dims, spacing = (20, 24, 28), (1.0, 1.5, 2.0)
+    W, H = 40, 36
+    view, c, S = _parallel_z_view(dims, spacing, W, H, scale=0.3 * dims[1] * spacing[1])
+    hu = 250
+    vol = np.full(dims[::-1], hu, np.int16)
+    drr = {"intercept": 0.02, "slope": 2e-5, "offset": 1.0, "output": orc.DRR_F32_L, "ref_termination": 0}
+    mu = 0.02 + 2e-5 * hu
+    jit = orc.jitter_table()
+    for jitter in (0, 1):
+        march = {"step_scale": 0.5, "jitter": jitter, "skip_rule": 1, "precise": 1}
+        L = orc.drr(vol, spacing, configs.view_dict(view), march, drr, "f64")
+        ns = (dims[2] - 3) * spacing[2] / (min(spacing) * 0.5)
+        ds = min(spacing) * 0.5
+        aspect = W / H
+        checked = 0
+        for y in range(H):
+            for x in range(W):
+                vx = (c[0] + (2.0 * x / W - 1.0) * S * aspect) / spacing[0]
+                vy = (c[1] + (2.0 * y / H - 1.0) * S) / spacing[1]
+                if not (1.01 < vx < dims[0] - 2.01 and 1.01 < vy < dims[1] - 2.01):
+                    continue
+                j = float(jit[(x % 16) + 16 * (y % 16)]) if jitter else 0.0
+                expect = math.floor(ns - j) * ds * mu
+                assert L[y, x] == pytest.approx(expect, rel=1e-6), (x, y)
+                checked += 1
+        assert checked > W * H // 2
+
+
+def test_drr_sphere_chord_lengths():
+    n = 64
+    c = (n - 1) / 2.0
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    R = 22.0
+    inside = (i - c) ** 2 + (j - c) ** 2 + (k - c) ** 2 <= R * R
+    vol = np.where(inside, 0.02, 0.0).astype(np.float32)
+    W = H = 64
+    view, cw, S = _parallel_z_view((n, n, n), (1, 1, 1), W, H, scale=0.5 * n)
+    drr = {"intercept": 0.0, "slope": 1.0, "offset": 1.0, "output": orc.DRR_F32_L, "ref_termination": 0}
+    march = {"step_scale": 0.25, "jitter": 0, "skip_rule": 0, "precise": 1}
+    L = orc.drr(vol, (1, 1, 1), configs.view_dict(view), march, drr, "f64")
+    errs = []
+    for y in range(8, 56, 4):
+        for x in range(8, 56, 4):
+            wx = cw[0] + (2.0 * x / W - 1.0) * S
+            wy = cw[1] + (2.0 * y / H - 1.0) * S
+            # sample p reads the voxel grid at p - 0.5: the sphere centre sits at c + 0.5 in ray space
+            r2 = (wx - 0.5 - c) ** 2 + (wy - 0.5 - c) ** 2
+            chord = 2.0 * math.sqrt(max(0.0, R * R - r2))
+            if chord > 20:
+                errs.append(abs(L[y, x] - 0.02 * chord) / (0.02 * chord))
+    # a voxelised sphere: the trilinear boundary blur stays within about one voxel of chord
+    assert len(errs) > 50 and max(errs) < 0.08 and np.mean(errs) < 0.02
+
+
+def test_drr_rgba8_reference_output_formula():
+    """A = tint.a + (1 - tint.a/255) * max(0, 256 - 256*offset*exp(-L))."""
+    s = configs.scene_c2(scale=0.125)
+    vd = configs.view_dict(s.views[0])
+    base = dict(s.drr, offset=0.8, ref_termination=0)
+    L = orc.drr(s.volume, s.spacing, vd, s.march, dict(base, output=orc.DRR_F32_L), "f64")
+    rgba = orc.drr(s.volume, s.spacing, vd, s.march, dict(base, output=orc.DRR_RGBA8_REF, tint=(10, 20, 30, 60)),
+                   "f64")
+    out = np.maximum(0.0, 256.0 - np.exp(math.log(256.0 * 0.8) - L))
+    A = 60 + (1 - 60 / 255.0) * out
+    expect = np.clip(np.floor(A), 0, 255)
+    assert np.mean(np.abs(rgba[..., 3].astype(float) - expect) <= 1) > 0.999
+    q = 60.0 / (60.0 + out)
+    np.testing.assert_allclose(rgba[..., 0], np.clip(np.floor(q * 10), 0, 255), atol=1)
+
+
+def test_drr_reference_termination_clamps_dense_rays():
+    s = configs.scene_c2(scale=0.125)
+    vd = configs.view_dict(s.views[0])
+    d = dict(s.drr, output=orc.DRR_F32_L, offset=0.05)
+    free = orc.drr(s.volume, s.spacing, vd, s.march, d, "f64")
+    stop = orc.drr(s.volume, s.spacing, vd, s.march, dict(d, ref_termination=1), "f64")
+    lim = math.log(256 * 0.05)
+    assert np.all(stop <= free + 1e-12)
+    dense = free > lim + 0.5
+    assert dense.any()
+    assert np.all(stop[dense] >= lim - 1e-9) and np.all(stop[dense] < lim + 0.25)
+
+
+# ---------------------------------------------------------------------------
+# naive second restatement of the DRR march (read from
+# V/CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo.cuh:11-117 independently of
+# oracle/rvk_oracle.cpp): positions by repeated addition, skip rule as a
+# small state machine over (backstep pending, skip taken).
+def _naive_sample(vol, p):
+    nz, ny, nx = vol.shape
+    q = [p[0] - 0.5, p[1] - 0.5, p[2] - 0.5]
+    lo = [math.floor(v) for v in q]
+    f = [q[a] - lo[a] for a in range(3)]
+    tot = 0.0
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                w = (f[0] if dx else 1 - f[0]) * (f[1] if dy else 1 - f[1]) * (f[2] if dz else 1 - f[2])
+                ix = min(max(lo[0] + dx, 0), nx - 1)
+                iy = min(max(lo[1] + dy, 0), ny - 1)
+                iz = min(max(lo[2] + dz, 0), nz - 1)
+                tot += w * float(vol[iz, iy, ix])
+    return tot
+
+
+def _naive_drr_L(vol, spacing, start, inc, num_steps, jit, slope, intercept, skip):
+    ds = math.sqrt(sum((inc[a] * spacing[a]) ** 2 for a in range(3)))
+    p = [start[a] + jit * inc[a] for a in range(3)]
+    n = math.floor(num_steps - jit)
+    L = 0.0
+    pending_back = False  # a skip was taken and the landing sample has not been tested yet
+    back_now = False
+    while n > 0:
+        mu = slope * _naive_sample(vol, p) + intercept
+        contributes = mu > 0
+        if not skip:
+            if contributes:
+                L += ds * mu
+            p = [p[a] + inc[a] for a in range(3)]
+            n -= 1
+            continue
+        if contributes:
+            back_now = pending_back
+            pending_back = False
+            if back_now:
+                p = [p[a] - inc[a] for a in range(3)]
+                n += 1
+            else:
+                p = [p[a] + inc[a] for a in range(3)]
+                n -= 1
+                L += ds * mu
+        else:
+            if not back_now:
+                p = [p[a] + inc[a] for a in range(3)]
+                n -= 1
+            pending_back = not back_now
+            p = [p[a] + inc[a] for a in range(3)]
+            n -= 1
+            back_now = False
+    return L
+
+
+@pytest.mark.parametrize("skip", [0, 1])
+def test_drr_matches_naive_restatement(skip):
+    s = configs.scene_c2(scale=0.05)  # 13^3
+    view = abi.compute_view(s.extra["camera"], 12, 10, spacing=s.spacing)
+    vd = configs.view_dict(view)
+    march = {"step_scale": 0.5, "jitter": 1, "skip_rule": skip, "precise": 0}
+    drr = {"intercept": 0.02, "slope": 2e-5, "offset": 1.0, "output": orc.DRR_F32_L, "ref_termination": 0}
+    L = orc.drr(s.volume, s.spacing, vd, march, drr, "f64")
+    rays = orc.form_rays(s.dims, s.spacing, vd, march, "f64")
+    jt = orc.jitter_table()
+    sl, ic = np.float32(2e-5), np.float32(0.02)
+    for y in range(view.height):
+        for x in range(view.width):
+            ref = _naive_drr_L(s.volume, s.spacing, rays["start"][y, x], rays["inc"][y, x], rays["num_steps"][y, x],
+                               float(jt[(x % 16) + 16 * (y % 16)]), float(sl), float(ic), skip)
+            assert L[y, x] == pytest.approx(ref, rel=1e-9, abs=1e-12), (x, y)
+
+
+# ---------------------------------------------------------------------------
+# compositing closed forms
+def _uniform_1d(value, a, col, n=64):
+    tf = {"alpha": np.full(n, a, np.float32), "galpha": np.ones(n, np.float32),
+          "r": np.full(n, col[0], np.float32), "g": np.full(n, col[1], np.float32),
+          "b": np.full(n, col[2], np.float32),
+          "params": {"intensity_low": 0.0, "intensity_multiplier": 1.0 / 255.0, "gradient_low": 0.0,
+                     "gradient_multiplier": float("inf"), "ambient": 1.0, "diffuse": 0.0, "specular": 0.0,
+                     "specular_power": 1.0}}
+    return tf
+
+
+@pytest.mark.parametrize("term", [0.0, 1.0 / 64])
+def test_1d_uniform_medium_closed_form(term):
+    dims, spacing = (16, 18, 20), (1.0, 1.0, 1.0)
+    W, H = 24, 20
+    view, _, _ = _parallel_z_view(dims, spacing, W, H, scale=0.25 * dims[1])
+    vol = np.full(dims[::-1], 120, np.uint8)
+    a, col = 0.15, (0.9, 0.5, 0.2)
+    tf = _uniform_1d(120, a, col)
+    march = {"step_scale": 0.5, "term_T": term, "jitter": 0, "skip_rule": 1, "precise": 1}
+    _, f, depth = orc.cast1d(vol, spacing, tf, configs.view_dict(view), march, "f64")
+    ns = math.floor((dims[2] - 3) / 0.5)
+    m = ns
+    if term > 0:
+        m = next(i for i in range(1, ns + 1) if (1 - a) ** i < term)
+    T = (1 - a) ** m
+    a32, c32 = float(np.float32(a)), [float(np.float32(v)) for v in col]  # tables hold float32
+    T = (1 - a32) ** m
+    for y in range(4, 16):
+        for x in range(6, 18):
+            assert f[y, x, 0] == pytest.approx(c32[0] * (1 - T), rel=1e-12)
+            assert f[y, x, 2] == pytest.approx(c32[2] * (1 - T), rel=1e-12)
+            assert f[y, x, 3] == pytest.approx(1.0 if term > 0 else 1 - T, rel=1e-12)
+            assert depth[y, x] == m
+
+
+def test_1d_zero_and_full_opacity():
+    dims, spacing = (16, 16, 16), (1.0, 1.0, 1.0)
+    view, _, _ = _parallel_z_view(dims, spacing, 20, 20, scale=4.0)
+    vol = np.full(dims[::-1], 50, np.uint8)
+    vd = configs.view_dict(view)
+    march = {"step_scale": 0.5, "term_T": 1.0 / 64, "jitter": 1, "skip_rule": 1}
+    rgba, _, _ = orc.cast1d(vol, spacing, _uniform_1d(50, 0.0, (1, 1, 1)), vd, march, "f64")
+    assert not rgba.any()
+    rgba, f, depth = orc.cast1d(vol, spacing, _uniform_1d(50, 1.0, (0.25, 0.5, 1.0)), vd, march, "f64")
+    hit = depth > 0
+    assert hit.sum() > 100
+    assert np.all(rgba[hit] == np.array([63, 127, 255, 255], np.uint8))
+    assert np.all(depth[hit] == 1)
+
+
+def test_2d_uniform_medium_closed_form():
+    dims, spacing = (16, 18, 20), (1.0, 1.0, 1.0)
+    view, _, _ = _parallel_z_view(dims, spacing, 24, 20, scale=0.25 * dims[1])
+    vol = np.full(dims[::-1], 1000, np.uint16)
+    N = 32
+    a = 0.05
+    tables = {k: np.zeros((N, N), np.float32) for k in orc.TF2D_NAMES}
+    tables["alpha"][0, :] = a  # gradient 0 -> gradient index 0
+    tables["red"][:] = 0.6
+    tables["ambient"][:] = 1.0
+    tables["specular_power"][:] = 1.0
+    params = configs.tf2d_params_for(0.0, 4095.0, 0.0, 100.0)
+    tf = {"tables": tables, "params": params}
+    march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 0, "skip_rule": 1}
+    _, f, depth = orc.cast2d(vol, spacing, tf, configs.view_dict(view), march, "f64")
+    m = math.floor((dims[2] - 3) / 0.5)
+    T = (1 - float(np.float32(a))) ** m
+    assert f[10, 12, 0] == pytest.approx(float(np.float32(0.6)) * (1 - T), rel=1e-12)
+    assert f[10, 12, 3] == pytest.approx(1 - T, rel=1e-12)
+
+
+def test_flip_symmetry():
+    s = configs.scene_c1(scale=0.25)
+    W, H = 48, 40
+    march = dict(s.march, jitter=0)
+    a = orc.cast1d(s.volume, s.spacing, s.tf1d, configs.view_dict(abi.compute_view(s.extra["camera"], W, H)), march,
+                   "f64")[1]
+    b = orc.cast1d(s.volume, s.spacing, s.tf1d,
+                   configs.view_dict(abi.compute_view(s.extra["camera"], W, H, flipped=True)), march, "f64")[1]
+    np.testing.assert_allclose(b[:, 1:], a[:, W - 1:0:-1], rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# precision
+def test_fp32_agrees_with_fp64_on_fp32_path():
+    s = configs.scene_c2(scale=0.25)
+    vd = configs.view_dict(s.views[0])
+    a = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f32")
+    b = orc.drr(s.volume, s.spacing, vd, dict(s.march, f32_control=1), s.drr, "f64")
+    assert np.max(np.abs(a - b) / b) < 1e-4
+    s1 = configs.scene_c1(scale=0.25)
+    vd1 = configs.view_dict(s1.views[0])
+    a8 = orc.cast1d(s1.volume, s1.spacing, s1.tf1d, vd1, s1.march, "f32")[0]
+    b8 = orc.cast1d(s1.volume, s1.spacing, s1.tf1d, vd1, dict(s1.march, f32_control=1), "f64")[0]
+    assert np.mean(np.abs(a8.astype(int) - b8.astype(int)).max(-1) <= 1) >= 0.999
+
+
+def test_sample_count_is_sum_of_step_counts():
+    s = configs.scene_c2(scale=0.125)
+    vd = configs.view_dict(s.views[0])
+    r = orc.form_rays(s.dims, s.spacing, vd, s.march, "f64")
+    jt = orc.jitter_table()
+    H, W = r["num_steps"].shape
+    j = np.array([[jt[(x % 16) + 16 * (y % 16)] for x in range(W)] for y in range(H)], np.float64)
+    expect = int(np.maximum(0, np.floor(r["num_steps"] - j)).sum())
+    assert orc.count_samples(s.volume, s.spacing, vd, s.march) == expect
+
+
+def test_post_shade_identity_at_zero_darkness():
+    rng = np.random.default_rng(3)
+    H, W = 20, 24
+    rgba = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    depth = rng.random((H, W)).astype(np.float32) * 50
+    mind = rng.random((H, W)).astype(np.float32)
+    maxd = (rng.random((H, W)) * 100 + 1).astype(np.float32)
+    shade = {"celr": 0.0, "cela": 0.002, "celc": 1 / 0.003, "disr": 0.0, "disa": 0.45, "disc": 1 / 0.15}
+    _, out = orc.post_shade(shade, depth, mind, maxd, rgba)
+    np.testing.assert_array_equal(out, rgba)

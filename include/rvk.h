@@ -207,6 +207,70 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
 int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
                    const float* min_depth, const float* max_depth, uint8_t* rgba);
 
+/* ---- batched DRR and its adjoint (no counterpart in the reference) ----------
+ * A plan holds many views of one volume frame -- rigid poses and detectors of
+ * different sizes -- and renders all of them in one launch.  Outputs are one
+ * packed float buffer: view v occupies W_v*H_v floats starting at
+ * rvk_drr_plan_offsets()[v].  Only the float output modes are batched.
+ *
+ * Work is cut into 8x8-pixel tiles, one per wave64.  Schedules:
+ *   RVK_SCHED_GRID        one wave per tile, tiles XCD-remapped;
+ *   RVK_SCHED_PERSISTENT  a grid sized to the device; each wave pulls tiles
+ *                         from per-XCD work queues and then steals from the
+ *                         other XCDs' queues until all are empty.
+ * rvk_drr_plan_autotune times the candidates on this plan and keeps the
+ * fastest; rvk_drr_plan_capture records the launch sequence in a hipGraph
+ * that rvk_drr_plan_render then replays. */
+typedef struct rvk_drr_plan rvk_drr_plan;
+
+typedef enum rvk_schedule { RVK_SCHED_GRID = 1, RVK_SCHED_PERSISTENT = 2 } rvk_schedule;
+
+/* dview (nullable): per view the 6 x 16 pose derivatives of view_to_voxels
+ * (rvk_posed_view's dm_out), needed by the adjoint. */
+int rvk_drr_plan_create(rvk_ctx* ctx, const rvk_volume* vol, int frame, int n_views, const rvk_view* views,
+                        const float* dview, const rvk_march_params* march, const rvk_drr_params* drr,
+                        rvk_drr_plan** out);
+int rvk_drr_plan_destroy(rvk_drr_plan* plan);
+/* Replace the views' matrices (and derivatives) keeping their sizes; a
+ * captured graph stays valid (it reads the views from device memory). */
+int rvk_drr_plan_update_views(rvk_drr_plan* plan, const rvk_view* views, const float* dview);
+int rvk_drr_plan_offsets(const rvk_drr_plan* plan, int64_t* offsets /* n_views + 1 */);
+int rvk_drr_plan_set_schedule(rvk_drr_plan* plan, int schedule, int waves_per_simd);
+int rvk_drr_plan_get_schedule(const rvk_drr_plan* plan, int* schedule, int* waves_per_simd);
+/* Time every candidate schedule `reps` times on this plan (writes into out)
+ * and keep the fastest; *best_ms (nullable) receives its time. */
+int rvk_drr_plan_autotune(rvk_drr_plan* plan, float* out, int reps, float* best_ms);
+/* Render every view into the packed buffer `out` on the context stream. */
+int rvk_drr_plan_render(rvk_drr_plan* plan, float* out);
+/* Capture render(out) (and, with targets, the registration step below) into
+ * a hipGraph; later rvk_drr_plan_render / rvk_drr_plan_register calls with
+ * the same buffers replay it.  Passing NULL drops the graph. */
+int rvk_drr_plan_capture(rvk_drr_plan* plan, float* out, const float* targets, float* loss, float* grad_pose);
+
+/* Zero-mean normalised cross-correlation of every view with packed target
+ * images: loss_v = 1 - NCC_v (n_views floats) and, if dloss_dimage is not
+ * NULL, d loss_v / d image (packed like the images).  The NCC is the centred
+ * form of vtkImageNormalizedCrossCorrelation::GetResult
+ * (ImageProcessing/vtkRobartsRegistration/vtkImageNormalizedCrossCorrelation.cxx:203-218,
+ * which is uncentred); a constant image or target gives NCC 0 and no gradient. */
+int rvk_drr_plan_ncc(rvk_drr_plan* plan, const float* images, const float* targets, float* loss,
+                     float* dloss_dimage);
+/* Adjoint of the batched render: given the rendered images and d loss / d
+ * image (packed), write d loss / d pose (n_views x 6 floats, pose as in
+ * rvk_pose_matrix; needs the plan's dview) and/or d loss / d volume (voxel
+ * values, x fastest; float32 volumes with the plain march only).  Rays are
+ * re-marched with the forward's control flow; nothing per sample is stored.
+ * The volume gradient is a voxel-driven gather (each voxel re-marches only
+ * the samples its trilinear support touches) with no atomics; its first call
+ * allocates a per-pixel ray record buffer inside the plan.  Gradients are
+ * those of the precise march. */
+int rvk_drr_plan_backward(rvk_drr_plan* plan, const float* images, const float* dloss_dimage, float* grad_pose,
+                          float* grad_volume);
+/* The registration step, fused: render every view into out, NCC against
+ * targets (loss, n_views floats) and the pose gradient of the loss (n_views x
+ * 6 floats), with d loss / d image formed inside the adjoint kernel. */
+int rvk_drr_plan_register(rvk_drr_plan* plan, float* out, const float* targets, float* loss, float* grad_pose);
+
 /* Ray-sample count of a plain march without opacity termination: sum over
  * pixels of floor(num_steps - ray offset) (the DRR's sample count, from the
  * geometry alone).  Synchronous; *out is a host pointer. */

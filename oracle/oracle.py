@@ -33,7 +33,8 @@ class OrcVolume(C.Structure):
 class OrcView(C.Structure):
     _fields_ = [("m", C.c_float * 16), ("width", C.c_int), ("height", C.c_int), ("flipped", C.c_int),
                 ("zbuffer", C.c_void_p), ("zb_width", C.c_int), ("zb_height", C.c_int),
-                ("n_clip", C.c_int), ("clip", C.c_float * 24), ("n_keyhole", C.c_int), ("keyhole", C.c_float * 24)]
+                ("n_clip", C.c_int), ("clip", C.c_float * 24), ("n_keyhole", C.c_int), ("keyhole", C.c_float * 24),
+                ("m64", C.c_void_p)]
 
 
 class OrcMarch(C.Structure):
@@ -126,11 +127,16 @@ def _view(view: dict) -> tuple[OrcView, object]:
     v.width, v.height = int(view["W"]), int(view["H"])
     v.flipped = int(view.get("flipped", 0))
     keep = None
+    if view.get("m64") is not None:
+        m64 = np.ascontiguousarray(view["m64"], np.float64).ravel()
+        v.m64 = m64.ctypes.data
+        keep = [m64]
     zb = view.get("zbuffer")
     if zb is not None:
-        keep = np.ascontiguousarray(zb, np.float32)
-        v.zbuffer = keep.ctypes.data
-        v.zb_height, v.zb_width = keep.shape
+        zba = np.ascontiguousarray(zb, np.float32)
+        keep = (keep or []) + [zba]
+        v.zbuffer = zba.ctypes.data
+        v.zb_height, v.zb_width = zba.shape
     clip = view.get("clip") or []
     key = view.get("keyhole") or []
     v.n_clip = len(clip)

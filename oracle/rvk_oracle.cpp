@@ -273,7 +273,7 @@ template <typename T>
 Ray<T> form_ray(const VolInfo<T>& vi, const orc_view* v, T step_len, int x, int y) {
   Ray<T> r;
   T m[16];
-  for (int i = 0; i < 16; ++i) m[i] = (T)v->m[i];
+  for (int i = 0; i < 16; ++i) m[i] = (v->m64 && sizeof(T) == sizeof(double)) ? (T)v->m64[i] : (T)v->m[i];
   const T vx = v->flipped ? (T)x / (T)v->width : T(1) - (T)x / (T)v->width;
   const T vy = (T)y / (T)v->height;
   T end_depth = T(1);

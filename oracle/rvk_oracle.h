@@ -51,6 +51,8 @@ typedef struct orc_view {
   float clip[24];
   int n_keyhole;       /* 0..6 keyhole planes in voxel space */
   float keyhole[24];
+  const double* m64;   /* nullable: the f64 flavour uses this matrix instead of m
+                          (finite differences of the pose need it unrounded) */
 } orc_view;
 
 typedef struct orc_march {

@@ -101,6 +101,19 @@ _SIGS = {
     "rvk_render_drr": (_I, [_P, _P, _I, C.POINTER(View), C.POINTER(MarchParams), C.POINTER(DrrParams), _P]),
     "rvk_post_shade": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
     "rvk_count_ray_steps": (_I, [_P, _P, C.POINTER(View), C.POINTER(MarchParams), C.POINTER(C.c_int64)]),
+    "rvk_drr_plan_create": (_I, [_P, _P, _I, _I, C.POINTER(View), _P, C.POINTER(MarchParams), C.POINTER(DrrParams),
+                                 C.POINTER(_P)]),
+    "rvk_drr_plan_destroy": (_I, [_P]),
+    "rvk_drr_plan_update_views": (_I, [_P, C.POINTER(View), _P]),
+    "rvk_drr_plan_offsets": (_I, [_P, C.POINTER(C.c_int64)]),
+    "rvk_drr_plan_set_schedule": (_I, [_P, _I, _I]),
+    "rvk_drr_plan_get_schedule": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "rvk_drr_plan_autotune": (_I, [_P, _P, _I, C.POINTER(C.c_float)]),
+    "rvk_drr_plan_render": (_I, [_P, _P]),
+    "rvk_drr_plan_capture": (_I, [_P, _P, _P, _P, _P]),
+    "rvk_drr_plan_ncc": (_I, [_P, _P, _P, _P, _P]),
+    "rvk_drr_plan_backward": (_I, [_P, _P, _P, _P, _P]),
+    "rvk_drr_plan_register": (_I, [_P, _P, _P, _P, _P]),
     "rvk_compute_view": (_I, [C.POINTER(Camera), _P, C.POINTER(C.c_double), C.POINTER(C.c_double), _I, _I, _I,
                               C.POINTER(View)]),
     "rvk_compute_matrices": (_I, [C.POINTER(Camera), _P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
@@ -330,6 +343,84 @@ def count_ray_steps(ctx: Context, vol: Volume, view: View, march: MarchParams) -
     n = C.c_int64(0)
     check(lib().rvk_count_ray_steps(ctx.handle, vol.handle, C.byref(view), C.byref(march), C.byref(n)))
     return int(n.value)
+
+
+SCHED_GRID, SCHED_PERSISTENT = 1, 2
+
+
+class DrrPlan:
+    """rvk_drr_plan: many views of one volume frame rendered in one launch,
+    plus NCC, the pose/volume adjoint and the fused registration step.
+    Buffers are torch tensors (or device pointers); images are packed."""
+
+    def __init__(self, ctx: Context, vol: Volume, views, march: MarchParams, drr: DrrParams, dview=None,
+                 frame: int = 0):
+        n = len(views)
+        arr = (View * n)(*views)
+        self._dview = None if dview is None else np.ascontiguousarray(dview, np.float32).reshape(n, 96)
+        h = C.c_void_p()
+        check(lib().rvk_drr_plan_create(ctx.handle, vol.handle, frame, n, arr,
+                                        None if self._dview is None else self._dview.ctypes.data,
+                                        C.byref(march), C.byref(drr), C.byref(h)))
+        self.handle = h
+        self.n_views = n
+        self.sizes = [(v.height, v.width) for v in views]
+        off = (C.c_int64 * (n + 1))()
+        check(lib().rvk_drr_plan_offsets(h, off))
+        self.offsets = [int(o) for o in off]
+        self.total = self.offsets[-1]
+
+    def update_views(self, views, dview=None):
+        arr = (View * len(views))(*views)
+        dv = None if dview is None else np.ascontiguousarray(dview, np.float32).reshape(len(views), 96)
+        check(lib().rvk_drr_plan_update_views(self.handle, arr, None if dv is None else dv.ctypes.data))
+
+    def set_schedule(self, schedule: int, waves_per_simd: int = 4):
+        check(lib().rvk_drr_plan_set_schedule(self.handle, schedule, waves_per_simd))
+
+    def schedule(self):
+        s, w = C.c_int(), C.c_int()
+        check(lib().rvk_drr_plan_get_schedule(self.handle, C.byref(s), C.byref(w)))
+        return s.value, w.value
+
+    def autotune(self, out, reps: int = 5) -> float:
+        ms = C.c_float()
+        check(lib().rvk_drr_plan_autotune(self.handle, C.c_void_p(dptr(out)), reps, C.byref(ms)))
+        return ms.value
+
+    def render(self, out):
+        check(lib().rvk_drr_plan_render(self.handle, C.c_void_p(dptr(out))))
+
+    def capture(self, out=None, targets=None, loss=None, grad_pose=None):
+        p = lambda t: None if t is None else C.c_void_p(dptr(t))
+        check(lib().rvk_drr_plan_capture(self.handle, p(out), p(targets), p(loss), p(grad_pose)))
+
+    def ncc(self, images, targets, loss, dloss_dimage=None):
+        p = lambda t: None if t is None else C.c_void_p(dptr(t))
+        check(lib().rvk_drr_plan_ncc(self.handle, p(images), p(targets), p(loss), p(dloss_dimage)))
+
+    def backward(self, images, dloss_dimage, grad_pose=None, grad_volume=None):
+        p = lambda t: None if t is None else C.c_void_p(dptr(t))
+        check(lib().rvk_drr_plan_backward(self.handle, p(images), p(dloss_dimage), p(grad_pose), p(grad_volume)))
+
+    def register(self, out, targets, loss, grad_pose):
+        p = lambda t: C.c_void_p(dptr(t))
+        check(lib().rvk_drr_plan_register(self.handle, p(out), p(targets), p(loss), p(grad_pose)))
+
+    def split(self, packed):
+        """Per-view [H, W] views of a packed torch tensor."""
+        return [packed[self.offsets[v]:self.offsets[v + 1]].view(*self.sizes[v]) for v in range(self.n_views)]
+
+    def close(self):
+        if self.handle:
+            lib().rvk_drr_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

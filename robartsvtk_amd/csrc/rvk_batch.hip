@@ -1,0 +1,886 @@
+// rvk_batch.hip -- batched multi-view DRR, NCC registration loss and the
+// hand-written DRR adjoint (pose and volume gradients).
+//
+// No counterpart in the reference: it renders one view per Render() call and
+// has no derivatives.  The forward march is the reference DRR caster
+// (V/CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo.cuh:11-117) shared with the
+// single-view kernel through rvk_march.h.
+//
+// Work unit: an 8x8-pixel tile of one view, processed by one wave64.  Tiles
+// are numbered view by view; inside a view, four consecutive tiles form a
+// 16x16 block (same ray-offset phase as the single-view kernel).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rvk_internal.h"
+#include "rvk_march.h"
+
+namespace rvk {
+
+struct BatchView {
+  ViewGeom v;
+  float dm[96];      // d(view_to_voxels)/d pose_i, i = 0..5 (baked, row major)
+  float inv[16];     // inverse of view_to_voxels: voxel position -> (vx, vy, z, 1) * w
+  long long pix_off; // first pixel of this view in the packed image buffer
+  int tiles16_x;
+  int pad;
+};
+
+struct BatchArgs {
+  VolGeom g;
+  DrrMarchArgs m;
+  float step_len;
+  const void* vol;
+  const BatchView* views;
+  const int* item_begin;  // n_views + 1 prefix sums of 8x8 tiles
+  int n_views;
+  int n_items;
+  float* out;
+  unsigned* queues;       // kNumXcd work-queue heads (persistent schedule)
+};
+
+__device__ __forceinline__ int find_view(const int* __restrict__ ib, int n, int item) {
+  int lo = 0, hi = n;  // ib[lo] <= item < ib[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (ib[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void item_pixel(int tiles16_x, int local, int lane, int& x, int& y) {
+  const int macro = local >> 2, q = local & 3;
+  x = (macro % tiles16_x) * 16 + (q & 1) * 8 + (lane & 7);
+  y = (macro / tiles16_x) * 16 + (q >> 1) * 8 + (lane >> 3);
+}
+
+template <typename VT, bool SKIP, bool PRECISE>
+__device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int lane) {
+  const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
+  const BatchView& bv = a.views[v];
+  int x, y;
+  item_pixel(bv.tiles16_x, item - a.item_begin[v], lane, x, y);
+  if (x >= bv.v.W || y >= bv.v.H) return;
+  const Ray r = form_ray(a.g, bv.v, a.step_len, x, y);
+  a.out[bv.pix_off + (long long)y * bv.v.W + x] = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a.g, a.m, r, x, y);
+}
+
+// One wave per tile; workgroups XCD-remapped so neighbouring tiles share an L2.
+template <typename VT, bool SKIP, bool PRECISE>
+__global__ __launch_bounds__(256) void batch_grid_kernel(BatchArgs a) {
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int item = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.n_items) return;
+  forward_item<VT, SKIP, PRECISE>(a, item, threadIdx.x & 63);
+}
+
+// Persistent: the tile range is cut into kNumXcd contiguous queues.  A wave
+// drains the queue of the XCD its workgroup was dispatched to (blockIdx mod
+// kNumXcd under round-robin dispatch), then the others in turn; every queue
+// is finite, so every wave reaches the end.
+template <typename VT, bool SKIP, bool PRECISE>
+__global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int home = blockIdx.x % kNumXcd;
+  for (int r = 0; r < kNumXcd; ++r) {
+    const int c = (home + r) % kNumXcd;
+    const int begin = (int)((long long)a.n_items * c / kNumXcd);
+    const int end = (int)((long long)a.n_items * (c + 1) / kNumXcd);
+    while (true) {
+      int w = 0;
+      if (lane == 0) w = (int)atomicAdd(&a.queues[c], 1u);
+      w = __builtin_amdgcn_readfirstlane(w);
+      const int item = begin + w;
+      if (item >= end) break;
+      forward_item<VT, SKIP, PRECISE>(a, item, lane);
+    }
+  }
+}
+
+// Queue heads are cleared by a kernel (not a memset node) so the sequence can
+// live in a replayed hipGraph.
+__global__ void reset_queues_kernel(unsigned* q) {
+  if (threadIdx.x < kNumXcd) q[threadIdx.x] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Zero-mean NCC.  Per tile: raw moments in double; per view: combine.
+struct NccArgs {
+  const BatchView* views;
+  const int* item_begin;
+  int n_views, n_items;
+  const float* img;
+  const float* tgt;
+  double* part;    // n_items x 5
+  double* stats;   // n_views x 8: mean_I, mean_T, inv_sqrt(Saa*Sbb), ncc/Saa, ncc, n, valid
+  float* loss;     // n_views
+  float* grad_img; // optional packed dloss/dI
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void ncc_partial_kernel(NccArgs a) {
+  const int item = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.n_items) return;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
+  const BatchView& bv = a.views[v];
+  int x, y;
+  item_pixel(bv.tiles16_x, item - a.item_begin[v], lane, x, y);
+  double s[5] = {0, 0, 0, 0, 0};
+  if (x < bv.v.W && y < bv.v.H) {
+    const long long i = bv.pix_off + (long long)y * bv.v.W + x;
+    const double I = a.img[i], T = a.tgt[i];
+    s[0] = I; s[1] = T; s[2] = I * I; s[3] = T * T; s[4] = I * T;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) s[k] = wave_sum(s[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a.part[(long long)item * 5 + k] = s[k];
+}
+
+// One workgroup per view; a fixed reduction order keeps results reproducible.
+__global__ __launch_bounds__(256) void ncc_view_kernel(NccArgs a) {
+  const int v = blockIdx.x;
+  __shared__ double red[5][256];
+  double s[5] = {0, 0, 0, 0, 0};
+  for (int it = a.item_begin[v] + threadIdx.x; it < a.item_begin[v + 1]; it += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s[k] += a.part[(long long)it * 5 + k];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n = (double)a.views[v].v.W * (double)a.views[v].v.H;
+    const double mi = red[0][0] / n, mt = red[1][0] / n;
+    const double saa = red[2][0] - n * mi * mi, sbb = red[3][0] - n * mt * mt, sab = red[4][0] - n * mi * mt;
+    const bool ok = saa > 0.0 && sbb > 0.0;
+    const double inv = ok ? 1.0 / sqrt(saa * sbb) : 0.0;
+    const double ncc = ok ? sab * inv : 0.0;
+    double* st = a.stats + 8 * v;
+    st[0] = mi; st[1] = mt; st[2] = inv; st[3] = ok ? ncc / saa : 0.0; st[4] = ncc; st[5] = n; st[6] = ok;
+    a.loss[v] = (float)(1.0 - ncc);
+  }
+}
+
+// d loss / d I_p = -( (T_p - mean_T) / sqrt(Saa Sbb) - ncc (I_p - mean_I) / Saa )
+__device__ __forceinline__ float dloss_dimage(const double* st, float I, float T) {
+  return (float)(-((double)T - st[1]) * st[2] + st[3] * ((double)I - st[0]));
+}
+
+__global__ __launch_bounds__(256) void ncc_grad_image_kernel(NccArgs a) {
+  const int item = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.n_items) return;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
+  const BatchView& bv = a.views[v];
+  int x, y;
+  item_pixel(bv.tiles16_x, item - a.item_begin[v], lane, x, y);
+  if (x >= bv.v.W || y >= bv.v.H) return;
+  const long long i = bv.pix_off + (long long)y * bv.v.W + x;
+  a.grad_img[i] = dloss_dimage(a.stats + 8 * v, a.img[i], a.tgt[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Pose adjoint.  For one pixel, L = sum_m ds*mu(p_m) over accumulated samples
+// p_m = start + (j + t_m) inc, so
+//   dL/dpose_i = A . dstart_i + B . dinc_i,
+//   A = sum_m ds*slope*grad mu(p_m),   B = sum_m (j + t_m) ds*slope*grad mu(p_m),
+// with (dstart_i, dinc_i) the forward-mode tangent of the ray set-up along
+// d(view_to_voxels)/d pose_i.  The ray is re-marched (PRECISE positions) with
+// the forward's control flow; nothing is stored per sample.
+struct AdjArgs {
+  BatchArgs b;
+  const float* img;   // rendered images (packed)
+  const float* dldi;  // d loss / d image, or NULL: formed from the NCC state
+  const double* stats;
+  const float* tgt;
+  float* part;  // n_items x 6
+};
+
+__device__ __forceinline__ float dloss_at(const float* dldi, const double* stats, const float* img, const float* tgt,
+                                          int v, long long pi) {
+  return dldi ? dldi[pi] : dloss_dimage(stats + 8 * v, img[pi], tgt[pi]);
+}
+
+template <typename VT>
+struct PoseVisitT {
+  const VT* vol;
+  const VolGeom& g;
+  float ncs, j;
+  V3<float>& A;
+  V3<float>& B;
+  __device__ __forceinline__ void operator()(const V3<float>& q, float t, float) const {
+    const V3<float> gr = trilinear_grad(vol, g, q.x, q.y, q.z);
+    const float w = j + t;
+    A.x += ncs * gr.x; A.y += ncs * gr.y; A.z += ncs * gr.z;
+    B.x += w * ncs * gr.x; B.y += w * ncs * gr.y; B.z += w * ncs * gr.z;
+  }
+};
+
+template <typename VT, bool SKIP>
+__global__ __launch_bounds__(256) void adjoint_pose_kernel(AdjArgs a) {
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int item = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.b.n_items) return;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(find_view(a.b.item_begin, a.b.n_views, item));
+  const BatchView& bv = a.b.views[v];
+  int x, y;
+  item_pixel(bv.tiles16_x, item - a.b.item_begin[v], lane, x, y);
+  float c[6] = {0, 0, 0, 0, 0, 0};
+  if (x < bv.v.W && y < bv.v.H) {
+    const VolGeom& g = a.b.g;
+    const Ray r = form_ray(g, bv.v, a.b.step_len, x, y);
+    const float ds = sqrtf(r.inc.x * r.inc.x * g.sp[0] * g.sp[0] + r.inc.y * r.inc.y * g.sp[1] * g.sp[1] +
+                           r.inc.z * r.inc.z * g.sp[2] * g.sp[2]);
+    V3<float> A{0, 0, 0}, B{0, 0, 0};
+    const float jt = ray_offset(a.b.m.jitter, x, y);
+    PoseVisitT<VT> visit{(const VT*)a.b.vol, g, ds * a.b.m.slope, jt, A, B};
+    const float I = drr_march<VT, SKIP, true>((const VT*)a.b.vol, g, a.b.m, r, x, y, visit);
+    const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
+    const float dLdI = dloss_at(a.dldi, a.stats, a.img, a.tgt, v, pi);
+    // dI/dL = -I for exp(-L) output, 1 for L output
+    const float coef = (a.b.m.output == RVK_DRR_F32_EXP_NEG_L) ? -dLdI * I : dLdI;
+    if (coef != 0.0f && (A.x != 0.0f || A.y != 0.0f || A.z != 0.0f)) {
+#pragma unroll 1
+      for (int i = 0; i < 6; ++i) {
+        const RayTangent tg = form_ray_tangent(g, bv.v, bv.dm + 16 * i, a.b.step_len, x, y);
+        c[i] = coef * (A.x * tg.dstart.x + A.y * tg.dstart.y + A.z * tg.dstart.z + B.x * tg.dinc.x +
+                       B.y * tg.dinc.y + B.z * tg.dinc.z);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) c[i] = wave_sumf(c[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) a.part[(long long)item * 6 + i] = c[i];
+}
+
+__global__ __launch_bounds__(256) void pose_reduce_kernel(const int* item_begin, const float* part, float* grad) {
+  const int v = blockIdx.x;
+  __shared__ double red[6][256];
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int it = item_begin[v] + threadIdx.x; it < item_begin[v + 1]; it += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] += part[(long long)it * 6 + k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) grad[6 * v + threadIdx.x] = (float)red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// Volume adjoint (f32 mu volumes, plain march): per pixel a ray record, then
+// a voxel-driven gather.  d loss / d mu_ijk = sum_views sum_pixels coef_p *
+// ds * sum_m [mu(p_m) > 0] w_ijk(p_m), w the trilinear weight of voxel ijk at
+// the sample.  Each voxel projects its support (p in (i-0.5, i+1.5)^3) onto
+// every detector, intersects the covered rays with the support, and re-marches
+// only those samples; one thread owns one voxel, so there are no atomics.
+struct RayRec {
+  float p0x, p0y, p0z, ix, iy, iz;
+  int n;
+  float cs, ci;   // alpha = cs * value + ci
+  float coef;     // d loss / d L_p * ds
+};
+
+struct VolGradArgs {
+  BatchArgs b;
+  const float* img;
+  const float* dldi;
+  RayRec* rays;   // packed like the images
+  float* grad;    // dims x-fastest
+};
+
+__global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
+  const int item = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.b.n_items) return;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(find_view(a.b.item_begin, a.b.n_views, item));
+  const BatchView& bv = a.b.views[v];
+  int x, y;
+  item_pixel(bv.tiles16_x, item - a.b.item_begin[v], lane, x, y);
+  if (x >= bv.v.W || y >= bv.v.H) return;
+  const VolGeom& g = a.b.g;
+  const Ray r = form_ray(g, bv.v, a.b.step_len, x, y);
+  const float j = ray_offset(a.b.m.jitter, x, y);
+  const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
+  const float I = a.img[pi];
+  const float dLdI = a.dldi[pi];
+  const float ds = sqrtf(r.inc.x * r.inc.x * g.sp[0] * g.sp[0] + r.inc.y * r.inc.y * g.sp[1] * g.sp[1] +
+                         r.inc.z * r.inc.z * g.sp[2] * g.sp[2]);
+  RayRec rr;
+  rr.p0x = r.start.x + j * r.inc.x;
+  rr.p0y = r.start.y + j * r.inc.y;
+  rr.p0z = r.start.z + j * r.inc.z;
+  rr.ix = r.inc.x; rr.iy = r.inc.y; rr.iz = r.inc.z;
+  rr.n = step_count(r.num_steps - j);
+  rr.ci = a.b.m.intercept * -ds;
+  rr.cs = a.b.m.slope * -ds;
+  // d loss / d mu at a sample = (d loss / d L) * ds * slope, with dL/dI = -I for exp(-L)
+  const float dLdL = (a.b.m.output == RVK_DRR_F32_EXP_NEG_L) ? -dLdI * I : dLdI;
+  rr.coef = dLdL * ds * a.b.m.slope;
+  a.rays[pi] = rr;
+}
+
+__device__ __forceinline__ float hat(float t) { return fmaxf(0.0f, 1.0f - fabsf(t)); }
+
+__global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
+  const VolGeom& g = a.b.g;
+  const long long vid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nvox = (long long)g.n[0] * g.n[1] * g.n[2];
+  if (vid >= nvox) return;
+  const int i = (int)(vid % g.n[0]);
+  const int jj = (int)((vid / g.n[0]) % g.n[1]);
+  const int k = (int)(vid / ((long long)g.n[0] * g.n[1]));
+  const float* vol = (const float*)a.b.vol;
+  // With mu = value (slope > 0, intercept 0) a sample is accumulated exactly
+  // when its trilinear value is > 0.  Samples in this voxel's support read
+  // cells inside [i-1, i+1]^3, so if every value there is <= 0 none is.
+  bool live = true;
+  if (a.b.m.intercept == 0.0f && a.b.m.slope > 0.0f) {
+    float mx = -1.0f;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int xi = min(max(i + dx, 0), g.n[0] - 1), yi = min(max(jj + dy, 0), g.n[1] - 1);
+          const int zi = min(max(k + dz, 0), g.n[2] - 1);
+          mx = fmaxf(mx, vol[brick_x(xi) + brick_y(yi, g.bys) + brick_z(zi, g.bzs)]);
+        }
+    live = mx > 0.0f;
+  }
+  float acc = 0.0f;
+  if (live) {
+    const float lo[3] = {(float)i - 0.5f, (float)jj - 0.5f, (float)k - 0.5f};
+    for (int v = 0; v < a.b.n_views; ++v) {
+      const BatchView& bv = a.b.views[v];
+      const int W = bv.v.W, H = bv.v.H;
+      // pixel box of the support's 8 corners
+      float xmin = 1e30f, xmax = -1e30f, ymin = 1e30f, ymax = -1e30f;
+      bool behind = false;
+      for (int c = 0; c < 8; ++c) {
+        const float px = lo[0] + ((c & 1) ? 2.0f : 0.0f), py = lo[1] + ((c & 2) ? 2.0f : 0.0f);
+        const float pz = lo[2] + ((c & 4) ? 2.0f : 0.0f);
+        const float* M = bv.inv;
+        const float hx = M[0] * px + M[1] * py + M[2] * pz + M[3];
+        const float hy = M[4] * px + M[5] * py + M[6] * pz + M[7];
+        const float hw = M[12] * px + M[13] * py + M[14] * pz + M[15];
+        if (!(hw > 0.0f)) { behind = true; break; }
+        const float vx = hx / hw, vy = hy / hw;
+        const float xp = bv.v.flipped ? vx * (float)W : (1.0f - vx) * (float)W;
+        const float yp = vy * (float)H;
+        xmin = fminf(xmin, xp); xmax = fmaxf(xmax, xp);
+        ymin = fminf(ymin, yp); ymax = fmaxf(ymax, yp);
+      }
+      int x0 = 0, x1 = W - 1, y0 = 0, y1 = H - 1;
+      if (!behind) {
+        x0 = max(0, (int)floorf(xmin) - 1); x1 = min(W - 1, (int)ceilf(xmax) + 1);
+        y0 = max(0, (int)floorf(ymin) - 1); y1 = min(H - 1, (int)ceilf(ymax) + 1);
+      }
+      for (int py = y0; py <= y1; ++py)
+        for (int px = x0; px <= x1; ++px) {
+          const RayRec rr = a.rays[bv.pix_off + (long long)py * W + px];
+          if (rr.n <= 0 || rr.coef == 0.0f) continue;
+          // sample indices m with p0 + m inc inside the open support box
+          float tlo = 0.0f, thi = (float)(rr.n - 1);
+          const float p0[3] = {rr.p0x, rr.p0y, rr.p0z}, in[3] = {rr.ix, rr.iy, rr.iz};
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax) {
+            if (in[ax] != 0.0f) {
+              const float ta = (lo[ax] - p0[ax]) / in[ax], tb = (lo[ax] + 2.0f - p0[ax]) / in[ax];
+              tlo = fmaxf(tlo, fminf(ta, tb));
+              thi = fminf(thi, fmaxf(ta, tb));
+            } else if (p0[ax] <= lo[ax] || p0[ax] >= lo[ax] + 2.0f) {
+              thi = -1.0f;
+            }
+          }
+          const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(rr.n - 1, (int)ceilf(thi) + 1);
+          for (int m = m0; m <= m1; ++m) {
+            const float t = (float)m;
+            const float qx = rr.p0x + t * rr.ix, qy = rr.p0y + t * rr.iy, qz = rr.p0z + t * rr.iz;
+            const float w = hat(qx - 0.5f - (float)i) * hat(qy - 0.5f - (float)jj) * hat(qz - 0.5f - (float)k);
+            if (w <= 0.0f) continue;
+            const float alpha = rr.cs * trilinear(vol, g, qx, qy, qz) + rr.ci;
+            if (alpha < 0.0f) acc += rr.coef * w;
+          }
+        }
+    }
+  }
+  a.grad[vid] = acc;
+}
+
+// ---------------------------------------------------------------------------
+template <typename F> static int dispatch_vt_b(int dtype, F&& f) {
+  switch (dtype) {
+    case RVK_U8: return f((uint8_t*)nullptr);
+    case RVK_I8: return f((int8_t*)nullptr);
+    case RVK_U16: return f((uint16_t*)nullptr);
+    case RVK_I16: return f((int16_t*)nullptr);
+    case RVK_U32: return f((uint32_t*)nullptr);
+    case RVK_I32: return f((int32_t*)nullptr);
+    case RVK_F32: return f((float*)nullptr);
+    default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+  }
+}
+
+}  // namespace rvk
+
+using namespace rvk;
+
+struct rvk_drr_plan {
+  rvk_ctx* ctx = nullptr;
+  const rvk_volume* vol = nullptr;
+  int frame = 0;
+  int n_views = 0;
+  rvk_march_params march{};
+  rvk_drr_params drr{};
+  std::vector<int> item_begin;     // host copy
+  std::vector<long long> pix_off;  // n_views + 1
+  int n_items = 0;
+  BatchView* d_views = nullptr;
+  int* d_item_begin = nullptr;
+  unsigned* d_queues = nullptr;
+  double* d_part = nullptr;   // n_items x 5
+  double* d_stats = nullptr;  // n_views x 8
+  float* d_pose_part = nullptr;  // n_items x 6
+  RayRec* d_rays = nullptr;      // per pixel, allocated by the first volume gradient
+  int schedule = RVK_SCHED_GRID;
+  int waves_per_simd = 4;
+  int n_cu = 256;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // buffers the graph was captured with
+  float* g_out = nullptr;
+  const float* g_tgt = nullptr;
+  float* g_loss = nullptr;
+  float* g_grad = nullptr;
+};
+
+namespace {
+
+int fill_views(rvk_drr_plan* p, const rvk_view* views, const float* dview, std::vector<BatchView>& hv) {
+  hv.resize(p->n_views);
+  for (int v = 0; v < p->n_views; ++v) {
+    int rc = check_view(&views[v]);
+    if (rc) return rc;
+    if (v < (int)p->pix_off.size() - 1 && p->item_begin.size() == (size_t)p->n_views + 1) {
+      const long long npix = (long long)views[v].width * views[v].height;
+      if (npix != p->pix_off[v + 1] - p->pix_off[v])
+        return fail(RVK_ERR_INVALID_ARGUMENT, "view %d changed size", v);
+    }
+    BatchView& b = hv[v];
+    std::memset(&b, 0, sizeof(b));
+    b.v = view_geom(&views[v]);
+    if (dview) std::memcpy(b.dm, dview + 96 * (size_t)v, sizeof(b.dm));
+    double m[16], mi[16];
+    for (int k = 0; k < 16; ++k) m[k] = views[v].view_to_voxels[k];
+    // inverse of the baked matrix (Gauss-Jordan, partial pivoting)
+    double aug[4][8];
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) { aug[r][c] = m[4 * r + c]; aug[r][4 + c] = (r == c); }
+    bool singular = false;
+    for (int c = 0; c < 4 && !singular; ++c) {
+      int pv = c;
+      for (int r = c + 1; r < 4; ++r)
+        if (std::fabs(aug[r][c]) > std::fabs(aug[pv][c])) pv = r;
+      if (aug[pv][c] == 0.0) { singular = true; break; }
+      for (int k = 0; k < 8; ++k) std::swap(aug[c][k], aug[pv][k]);
+      const double iv = 1.0 / aug[c][c];
+      for (int k = 0; k < 8; ++k) aug[c][k] *= iv;
+      for (int r = 0; r < 4; ++r)
+        if (r != c && aug[r][c] != 0.0) {
+          const double f = aug[r][c];
+          for (int k = 0; k < 8; ++k) aug[r][k] -= f * aug[c][k];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) mi[4 * r + c] = singular ? 0.0 : aug[r][4 + c];
+    for (int k = 0; k < 16; ++k) b.inv[k] = (float)mi[k];
+    b.tiles16_x = (views[v].width + 15) / 16;
+  }
+  return RVK_OK;
+}
+
+BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
+  BatchArgs a;
+  a.g = vol_geom(p->vol);
+  a.m.intercept = p->drr.ct_intercept;
+  a.m.slope = p->drr.ct_slope;
+  a.m.offset = p->drr.ct_offset;
+  a.m.output = p->drr.output;
+  a.m.ref_term = p->drr.ref_termination;
+  a.m.jitter = p->march.jitter;
+  a.step_len = min_spacing(p->vol) * p->march.step_scale;
+  a.vol = p->vol->frames[p->frame];
+  a.views = p->d_views;
+  a.item_begin = p->d_item_begin;
+  a.n_views = p->n_views;
+  a.n_items = p->n_items;
+  a.out = out;
+  a.queues = p->d_queues;
+  return a;
+}
+
+int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s) {
+  BatchArgs a = batch_args(p, out);
+  const bool skip = p->march.skip_rule != 0, pr = p->march.precise != 0;
+  return dispatch_vt_b(p->vol->dtype, [&](auto tag) {
+    using VT = std::remove_pointer_t<decltype(tag)>;
+    if (p->schedule == RVK_SCHED_PERSISTENT) {
+      hipLaunchKernelGGL(reset_queues_kernel, dim3(1), dim3(64), 0, s, p->d_queues);
+      const int blocks = std::max(1, std::min(p->n_cu * p->waves_per_simd, (p->n_items + 3) / 4));
+#define RVK_LP(S, P) hipLaunchKernelGGL((batch_persistent_kernel<VT, S, P>), dim3(blocks), dim3(256), 0, s, a)
+      if (skip) { if (pr) RVK_LP(true, true); else RVK_LP(true, false); }
+      else { if (pr) RVK_LP(false, true); else RVK_LP(false, false); }
+#undef RVK_LP
+    } else {
+      const int blocks = (p->n_items + 3) / 4;
+#define RVK_LG(S, P) hipLaunchKernelGGL((batch_grid_kernel<VT, S, P>), dim3(blocks), dim3(256), 0, s, a)
+      if (skip) { if (pr) RVK_LG(true, true); else RVK_LG(true, false); }
+      else { if (pr) RVK_LG(false, true); else RVK_LG(false, false); }
+#undef RVK_LG
+    }
+    RVK_HIP(hipGetLastError());
+    return (int)RVK_OK;
+  });
+}
+
+NccArgs ncc_args(const rvk_drr_plan* p, const float* img, const float* tgt, float* loss, float* grad_img) {
+  NccArgs n;
+  n.views = p->d_views;
+  n.item_begin = p->d_item_begin;
+  n.n_views = p->n_views;
+  n.n_items = p->n_items;
+  n.img = img;
+  n.tgt = tgt;
+  n.part = p->d_part;
+  n.stats = p->d_stats;
+  n.loss = loss;
+  n.grad_img = grad_img;
+  return n;
+}
+
+int launch_register(rvk_drr_plan* p, float* out, const float* tgt, float* loss, float* grad, hipStream_t s) {
+  int rc = launch_forward(p, out, s);
+  if (rc) return rc;
+  const int blocks = (p->n_items + 3) / 4;
+  NccArgs n = ncc_args(p, out, tgt, loss, nullptr);
+  hipLaunchKernelGGL(ncc_partial_kernel, dim3(blocks), dim3(256), 0, s, n);
+  hipLaunchKernelGGL(ncc_view_kernel, dim3(p->n_views), dim3(256), 0, s, n);
+  AdjArgs aa;
+  aa.b = batch_args(p, out);
+  aa.img = out;
+  aa.dldi = nullptr;
+  aa.stats = p->d_stats;
+  aa.tgt = tgt;
+  aa.part = p->d_pose_part;
+  rc = dispatch_vt_b(p->vol->dtype, [&](auto tag) {
+    using VT = std::remove_pointer_t<decltype(tag)>;
+    if (p->march.skip_rule) hipLaunchKernelGGL((adjoint_pose_kernel<VT, true>), dim3(blocks), dim3(256), 0, s, aa);
+    else hipLaunchKernelGGL((adjoint_pose_kernel<VT, false>), dim3(blocks), dim3(256), 0, s, aa);
+    return (int)RVK_OK;
+  });
+  if (rc) return rc;
+  hipLaunchKernelGGL(pose_reduce_kernel, dim3(p->n_views), dim3(256), 0, s, p->d_item_begin, p->d_pose_part, grad);
+  RVK_HIP(hipGetLastError());
+  return RVK_OK;
+}
+
+void drop_graph(rvk_drr_plan* p) {
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  p->g_out = nullptr;
+  p->g_tgt = nullptr;
+  p->g_loss = nullptr;
+  p->g_grad = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rvk_drr_plan_create(rvk_ctx* ctx, const rvk_volume* vol, int frame, int n_views, const rvk_view* views,
+                        const float* dview, const rvk_march_params* march, const rvk_drr_params* drr,
+                        rvk_drr_plan** out) {
+  if (!ctx || !views || !march || !drr || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  int rc = check_frame(vol, frame);
+  if (rc) return rc;
+  if (n_views < 1 || n_views > 65536) return fail(RVK_ERR_INVALID_ARGUMENT, "n_views %d out of range", n_views);
+  if (drr->output != RVK_DRR_F32_EXP_NEG_L && drr->output != RVK_DRR_F32_L)
+    return fail(RVK_ERR_UNSUPPORTED, "batched DRR renders the float output modes only");
+  if (!(march->step_scale > 0.0f) || !std::isfinite(march->step_scale))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "step_scale must be positive and finite");
+  RVK_HIP(hipSetDevice(ctx->device));
+  rvk_drr_plan* p = new rvk_drr_plan();
+  p->ctx = ctx;
+  p->vol = vol;
+  p->frame = frame;
+  p->n_views = n_views;
+  p->march = *march;
+  p->drr = *drr;
+  p->item_begin.assign(n_views + 1, 0);
+  p->pix_off.assign(n_views + 1, 0);
+  long long items = 0;
+  for (int v = 0; v < n_views; ++v) {
+    p->item_begin[v] = (int)items;
+    p->pix_off[v] = (v ? p->pix_off[v - 1] + (long long)views[v - 1].width * views[v - 1].height : 0);
+    const long long t = 4LL * ((views[v].width + 15) / 16) * ((views[v].height + 15) / 16);
+    items += t;
+  }
+  p->pix_off[n_views] = p->pix_off[n_views - 1] + (long long)views[n_views - 1].width * views[n_views - 1].height;
+  if (items > (1LL << 30)) { delete p; return fail(RVK_ERR_INVALID_ARGUMENT, "too many pixels in one plan"); }
+  p->item_begin[n_views] = (int)items;
+  p->n_items = (int)items;
+  std::vector<BatchView> hv;
+  std::vector<long long> keep = p->pix_off;
+  p->pix_off.clear();  // no size check on creation
+  rc = fill_views(p, views, dview, hv);
+  p->pix_off = keep;
+  if (rc) { delete p; return rc; }
+  for (int v = 0; v < n_views; ++v) hv[v].pix_off = p->pix_off[v];
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) p->n_cu = prop.multiProcessorCount;
+  auto alloc = [&](void** d, size_t bytes) { return hipMalloc(d, bytes); };
+  hipError_t e = alloc((void**)&p->d_views, sizeof(BatchView) * n_views);
+  if (e == hipSuccess) e = alloc((void**)&p->d_item_begin, sizeof(int) * (n_views + 1));
+  if (e == hipSuccess) e = alloc((void**)&p->d_queues, sizeof(unsigned) * kNumXcd);
+  if (e == hipSuccess) e = alloc((void**)&p->d_part, sizeof(double) * 5 * (size_t)p->n_items);
+  if (e == hipSuccess) e = alloc((void**)&p->d_stats, sizeof(double) * 8 * (size_t)n_views);
+  if (e == hipSuccess) e = alloc((void**)&p->d_pose_part, sizeof(float) * 6 * (size_t)p->n_items);
+  if (e == hipSuccess) e = hipMemcpy(p->d_views, hv.data(), sizeof(BatchView) * n_views, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_item_begin, p->item_begin.data(), sizeof(int) * (n_views + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    rvk_drr_plan_destroy(p);
+    return hip_fail(e, "plan allocation");
+  }
+  *out = p;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_destroy(rvk_drr_plan* p) {
+  if (!p) return RVK_OK;
+  hipSetDevice(p->ctx->device);
+  hipStreamSynchronize(p->ctx->stream);
+  drop_graph(p);
+  if (p->cap_stream) hipStreamDestroy(p->cap_stream);
+  hipFree(p->d_views);
+  hipFree(p->d_item_begin);
+  hipFree(p->d_queues);
+  hipFree(p->d_part);
+  hipFree(p->d_stats);
+  hipFree(p->d_pose_part);
+  if (p->d_rays) hipFree(p->d_rays);
+  delete p;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_update_views(rvk_drr_plan* p, const rvk_view* views, const float* dview) {
+  if (!p || !views) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::vector<BatchView> hv;
+  int rc = fill_views(p, views, dview, hv);
+  if (rc) return rc;
+  for (int v = 0; v < p->n_views; ++v) hv[v].pix_off = p->pix_off[v];
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  RVK_HIP(hipMemcpyAsync(p->d_views, hv.data(), sizeof(BatchView) * p->n_views, hipMemcpyHostToDevice,
+                         p->ctx->stream));
+  RVK_HIP(hipStreamSynchronize(p->ctx->stream));
+  return RVK_OK;
+}
+
+int rvk_drr_plan_offsets(const rvk_drr_plan* p, int64_t* offsets) {
+  if (!p || !offsets) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  for (int v = 0; v <= p->n_views; ++v) offsets[v] = p->pix_off[v];
+  return RVK_OK;
+}
+
+int rvk_drr_plan_set_schedule(rvk_drr_plan* p, int schedule, int wps) {
+  if (!p) return fail(RVK_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (schedule != RVK_SCHED_GRID && schedule != RVK_SCHED_PERSISTENT)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown schedule %d", schedule);
+  if (wps < 1 || wps > 8) return fail(RVK_ERR_INVALID_ARGUMENT, "waves_per_simd must be within 1..8");
+  if (schedule != p->schedule || wps != p->waves_per_simd) drop_graph(p);
+  p->schedule = schedule;
+  p->waves_per_simd = wps;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_get_schedule(const rvk_drr_plan* p, int* schedule, int* wps) {
+  if (!p) return fail(RVK_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (schedule) *schedule = p->schedule;
+  if (wps) *wps = p->waves_per_simd;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms) {
+  if (!p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  reps = std::max(1, std::min(reps, 100));
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  drop_graph(p);
+  const int cand[][2] = {{RVK_SCHED_GRID, 4}, {RVK_SCHED_PERSISTENT, 2}, {RVK_SCHED_PERSISTENT, 4},
+                         {RVK_SCHED_PERSISTENT, 8}};
+  hipEvent_t e0, e1;
+  RVK_HIP(hipEventCreate(&e0));
+  RVK_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bs = p->schedule, bw = p->waves_per_simd;
+  int rc = RVK_OK;
+  for (const auto& c : cand) {
+    p->schedule = c[0];
+    p->waves_per_simd = c[1];
+    rc = launch_forward(p, out, p->ctx->stream);  // warm-up
+    if (rc) break;
+    hipEventRecord(e0, p->ctx->stream);
+    for (int r = 0; r < reps && !rc; ++r) rc = launch_forward(p, out, p->ctx->stream);
+    hipEventRecord(e1, p->ctx->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    if (ms < best) { best = ms; bs = c[0]; bw = c[1]; }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  p->schedule = bs;
+  p->waves_per_simd = bw;
+  if (best_ms) *best_ms = best;
+  return rc;
+}
+
+int rvk_drr_plan_render(rvk_drr_plan* p, float* out) {
+  if (!p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  if (p->exec && p->g_out == out && !p->g_tgt) {
+    RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
+    return RVK_OK;
+  }
+  return launch_forward(p, out, p->ctx->stream);
+}
+
+int rvk_drr_plan_capture(rvk_drr_plan* p, float* out, const float* tgt, float* loss, float* grad) {
+  if (!p) return fail(RVK_ERR_INVALID_ARGUMENT, "plan is NULL");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  drop_graph(p);
+  if (!out) return RVK_OK;
+  if (tgt && (!loss || !grad)) return fail(RVK_ERR_INVALID_ARGUMENT, "registration capture needs loss and grad");
+  if (tgt && !p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
+  // Capture on the plan's own stream (the context may use the legacy default
+  // stream, which cannot be captured); replays go to the context stream.
+  RVK_HIP(hipStreamSynchronize(p->ctx->stream));
+  RVK_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc = tgt ? launch_register(p, out, tgt, loss, grad, p->cap_stream) : launch_forward(p, out, p->cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  p->graph = g;
+  e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) { drop_graph(p); return hip_fail(e, "hipGraphInstantiate"); }
+  p->g_out = out;
+  p->g_tgt = tgt;
+  p->g_loss = loss;
+  p->g_grad = grad;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_register(rvk_drr_plan* p, float* out, const float* tgt, float* loss, float* grad) {
+  if (!p || !out || !tgt || !loss || !grad) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  if (p->exec && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad) {
+    RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
+    return RVK_OK;
+  }
+  return launch_register(p, out, tgt, loss, grad, p->ctx->stream);
+}
+
+int rvk_drr_plan_ncc(rvk_drr_plan* p, const float* images, const float* tgt, float* loss, float* dldi) {
+  if (!p || !images || !tgt || !loss) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  const int blocks = (p->n_items + 3) / 4;
+  NccArgs n = ncc_args(p, images, tgt, loss, dldi);
+  hipLaunchKernelGGL(ncc_partial_kernel, dim3(blocks), dim3(256), 0, p->ctx->stream, n);
+  hipLaunchKernelGGL(ncc_view_kernel, dim3(p->n_views), dim3(256), 0, p->ctx->stream, n);
+  if (dldi) hipLaunchKernelGGL(ncc_grad_image_kernel, dim3(blocks), dim3(256), 0, p->ctx->stream, n);
+  RVK_HIP(hipGetLastError());
+  return RVK_OK;
+}
+
+int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dldi, float* grad_pose,
+                          float* grad_vol) {
+  if (!p || !images || !dldi) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  hipStream_t s = p->ctx->stream;
+  const int blocks = (p->n_items + 3) / 4;
+  if (grad_pose) {
+    AdjArgs aa;
+    aa.b = batch_args(p, const_cast<float*>(images));
+    aa.img = images;
+    aa.dldi = dldi;
+    aa.stats = nullptr;
+    aa.tgt = nullptr;
+    aa.part = p->d_pose_part;
+    int rc = dispatch_vt_b(p->vol->dtype, [&](auto tag) {
+      using VT = std::remove_pointer_t<decltype(tag)>;
+      if (p->march.skip_rule) hipLaunchKernelGGL((adjoint_pose_kernel<VT, true>), dim3(blocks), dim3(256), 0, s, aa);
+      else hipLaunchKernelGGL((adjoint_pose_kernel<VT, false>), dim3(blocks), dim3(256), 0, s, aa);
+      return (int)RVK_OK;
+    });
+    if (rc) return rc;
+    hipLaunchKernelGGL(pose_reduce_kernel, dim3(p->n_views), dim3(256), 0, s, p->d_item_begin, p->d_pose_part,
+                       grad_pose);
+    RVK_HIP(hipGetLastError());
+  }
+  if (grad_vol) {
+    if (p->vol->dtype != RVK_F32)
+      return fail(RVK_ERR_UNSUPPORTED, "the volume gradient is defined for float32 volumes");
+    if (p->march.skip_rule || p->drr.ref_termination)
+      return fail(RVK_ERR_UNSUPPORTED, "the volume gradient needs the plain march (no skip rule, no termination)");
+    if (!p->d_rays) RVK_HIP(hipMalloc((void**)&p->d_rays, sizeof(RayRec) * (size_t)p->pix_off[p->n_views]));
+    VolGradArgs va;
+    va.b = batch_args(p, nullptr);
+    va.img = images;
+    va.dldi = dldi;
+    va.rays = p->d_rays;
+    va.grad = grad_vol;
+    hipLaunchKernelGGL(ray_record_kernel, dim3(blocks), dim3(256), 0, s, va);
+    const long long nvox = (long long)p->vol->dims[0] * p->vol->dims[1] * p->vol->dims[2];
+    hipLaunchKernelGGL(volume_grad_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, va);
+    RVK_HIP(hipGetLastError());
+  }
+  return RVK_OK;
+}
+
+}  // extern "C"

@@ -14,148 +14,20 @@
 #include <cmath>
 
 #include "rvk_internal.h"
+#include "rvk_march.h"
 
 namespace rvk {
 
-// The reference's 16x16 ray-offset table (V/vtkCudaVolumeMapper.cxx:59-93),
-// indexed by (x mod 16) + 16 (y mod 16).  Parity depends on it bit for bit.
-__constant__ float kRayOffsets[256] = {
-  0.70554f, 0.53342f, 0.57951f, 0.28956f, 0.30194f, 0.77474f, 0.01401f, 0.76072f,
-  0.81449f, 0.70903f, 0.04535f, 0.41403f, 0.86261f, 0.79048f, 0.37353f, 0.96195f,
-  0.87144f, 0.05623f, 0.94955f, 0.36401f, 0.52486f, 0.76711f, 0.0535f, 0.59245f,
-  0.4687f, 0.29816f, 0.62269f, 0.64782f, 0.26379f, 0.27934f, 0.8298f, 0.8246f,
-  0.58916f, 0.98609f, 0.91096f, 0.22686f, 0.69511f, 0.98f, 0.24393f, 0.53387f,
-  0.10636f, 0.99941f, 0.67617f, 0.0157f, 0.57518f, 0.10005f, 0.10302f, 0.79888f,
-  0.28448f, 0.04564f, 0.29577f, 0.38201f, 0.30097f, 0.94857f, 0.97982f, 0.40137f,
-  0.27827f, 0.16044f, 0.16282f, 0.64658f, 0.41007f, 0.41276f, 0.71273f, 0.3262f,
-  0.63317f, 0.20756f, 0.18601f, 0.58335f, 0.08071f, 0.45797f, 0.90572f, 0.26136f,
-  0.78521f, 0.3789f, 0.28966f, 0.91937f, 0.63174f, 0.62764f, 0.42845f, 0.09797f,
-  0.56104f, 0.69448f, 0.91371f, 0.83481f, 0.02262f, 0.54336f, 0.91616f, 0.43026f,
-  0.67794f, 0.50245f, 0.51373f, 0.46298f, 0.35347f, 0.40483f, 0.26973f, 0.05559f,
-  0.24384f, 0.97907f, 0.06091f, 0.39029f, 0.36499f, 0.48989f, 0.15566f, 0.47445f,
-  0.25726f, 0.62875f, 0.54207f, 0.1563f, 0.93854f, 0.65449f, 0.50608f, 0.39047f,
-  0.10737f, 0.78399f, 0.45964f, 0.75368f, 0.59609f, 0.83273f, 0.01875f, 0.21036f,
-  0.07395f, 0.10545f, 0.33169f, 0.12824f, 0.00024f, 0.53679f, 0.65705f, 0.54401f,
-  0.82741f, 0.08189f, 0.19192f, 0.67891f, 0.4542f, 0.35702f, 0.14998f, 0.70439f,
-  0.92878f, 0.53021f, 0.08964f, 0.75772f, 0.40184f, 0.46187f, 0.49216f, 0.20762f,
-  0.32973f, 0.09542f, 0.58979f, 0.16987f, 0.92761f, 0.09792f, 0.44386f, 0.27294f,
-  0.87254f, 0.75068f, 0.27294f, 0.67364f, 0.25662f, 0.08989f, 0.03095f, 0.32271f,
-  0.79012f, 0.29725f, 0.23528f, 0.48047f, 0.2546f, 0.3406f, 0.04493f, 0.48242f,
-  0.20601f, 0.86453f, 0.58862f, 0.7549f, 0.92788f, 0.33101f, 0.54294f, 0.08069f,
-  0.63437f, 0.41003f, 0.96042f, 0.11462f, 0.92344f, 0.6202f, 0.34772f, 0.14924f,
-  0.47997f, 0.2194f, 0.99373f, 0.13042f, 0.02888f, 0.34539f, 0.54766f, 0.92295f,
-  0.53824f, 0.40642f, 0.84724f, 0.82622f, 0.67242f, 0.72189f, 0.99677f, 0.3398f,
-  0.49521f, 0.41296f, 0.69528f, 0.17908f, 0.42291f, 0.54317f, 0.81466f, 0.54091f,
-  0.42753f, 0.50906f, 0.22778f, 0.61918f, 0.48983f, 0.68081f, 0.8866f, 0.37051f,
-  0.30249f, 0.29286f, 0.15031f, 0.52982f, 0.22326f, 0.58452f, 0.36345f, 0.87597f,
-  0.47801f, 0.19063f, 0.68406f, 0.74741f, 0.61393f, 0.78213f, 0.16174f, 0.80777f,
-  0.20261f, 0.95676f, 0.06585f, 0.06152f, 0.79319f, 0.3796f, 0.46358f, 0.11954f,
-  0.11547f, 0.17377f, 0.04811f, 0.71481f, 0.53302f, 0.561f, 0.21673f, 0.468f,
-  0.74635f, 0.75231f, 0.39893f, 0.90309f, 0.746f, 0.08855f, 0.63457f, 0.71302f};
-
-__device__ __forceinline__ float ray_offset(int jitter, int x, int y) {
-  return jitter ? kRayOffsets[(x & 15) + 16 * (y & 15)] : 0.0f;
-}
-
-__device__ __forceinline__ float ray_length(const VolGeom& g, const V3<float>& inc) {
-  return sqrtf(inc.x * inc.x * g.sp[0] * g.sp[0] + inc.y * inc.y * g.sp[1] * g.sp[1] +
-               inc.z * inc.z * g.sp[2] * g.sp[2]);
-}
-
-__device__ __forceinline__ void step(V3<float>& p, const V3<float>& inc) {
-  p.x += inc.x; p.y += inc.y; p.z += inc.z;
-}
-
-// Sample position.  Reference mode keeps the incrementally updated p
-// (p += inc per step, as the CUDA kernels do); PRECISE derives it from the
-// step index, p0 + (k_start - k) inc, which removes the drift that repeated
-// FP32 additions build up over long rays.
-template <bool PRECISE>
-__device__ __forceinline__ V3<float> cur_pos(const V3<float>& p, const V3<float>& p0, const V3<float>& inc, int kst,
-                                             int k) {
-  if (!PRECISE) return p;
-  const float t = (float)(kst - k);
-  return V3<float>{p0.x + t * inc.x, p0.y + t * inc.y, p0.z + t * inc.z};
-}
-
-// ===========================================================================
-// DRR
 struct DrrArgs {
   VolGeom g;
   ViewGeom v;
   float step_len;
-  int jitter, skip, ref_term;
-  float intercept, slope, offset;
-  int output;
+  DrrMarchArgs m;
   uint32_t tint;
   const void* vol;
   void* out;
   int tiles_x;
 };
-
-// Line integral along one ray (V/...DRR...cuh:11-117).  The accumulator starts
-// at ln(256*offset) in the reference RGBA8 mode and at 0 in the float modes.
-template <typename VT, bool SKIP, bool PRECISE>
-__device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const DrrArgs& a, const Ray& r, int x, int y) {
-  const bool ref_mode = a.output == RVK_DRR_RGBA8_REF;
-  const float lg = logf(256.0f * a.offset);
-  float acc = ref_mode ? lg : 0.0f;
-  const float stop_at = ref_mode ? 0.0f : -lg;
-  const float j = ray_offset(a.jitter, x, y);
-  int k = step_count(r.num_steps - j);
-  V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
-  const V3<float> inc = r.inc;
-  const float ds = ray_length(a.g, inc);
-  const float ci = a.intercept * -ds;
-  const float cs = a.slope * -ds;
-  const int es = k - (int)ceilf(r.exclude_start);
-  const int ee = k - (int)floorf(r.exclude_end);
-  const int kst = k;
-  const V3<float> p0 = p;
-  float comp = 0.0f;  // Kahan compensation of the line integral (PRECISE)
-  auto add = [&](float v) {
-    if (!PRECISE) { acc += v; return; }
-    const float yk = v - comp;
-    const float tk = acc + yk;
-    comp = (tk - acc) - yk;
-    acc = tk;
-  };
-  int sa = 0, sb = 0;
-  while (k > 0) {
-    const bool in_key = (es >= k && ee < k);
-    const V3<float> q = cur_pos<PRECISE>(p, p0, inc, kst, k);
-    const float alpha = cs * trilinear(vol, a.g, q.x, q.y, q.z) + ci;
-    if (SKIP) {
-      if (alpha < 0.0f && !in_key) {
-        sa = sb;
-        sb = 0;
-        if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
-        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
-        if (!sa) add(alpha);
-        if (a.ref_term && acc <= stop_at) break;
-      } else {
-        if (!sa) { step(p, inc); k--; }
-        sb = !sa;
-        step(p, inc);
-        k--;
-        sa = 0;
-      }
-    } else {
-      step(p, inc);
-      k--;
-      if (alpha < 0.0f && !in_key) {
-        add(alpha);
-        if (a.ref_term && acc <= stop_at) break;
-      }
-    }
-  }
-  if (ref_mode) {
-    const float o = 256.0f - expf(acc);
-    return o < 0.0f ? 0.0f : o;
-  }
-  if (a.output == RVK_DRR_F32_EXP_NEG_L) return expf(acc);
-  return -acc;
-}
 
 template <typename VT, bool SKIP, bool PRECISE>
 __global__ __launch_bounds__(256) void drr_kernel(DrrArgs a) {
@@ -164,9 +36,9 @@ __global__ __launch_bounds__(256) void drr_kernel(DrrArgs a) {
   tile_pixel(tile, a.tiles_x, x, y);
   if (x >= a.v.W || y >= a.v.H) return;
   const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
-  const float v = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a, r, x, y);
+  const float v = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a.g, a.m, r, x, y);
   const long idx = (long)y * a.v.W + x;
-  if (a.output == RVK_DRR_RGBA8_REF) {
+  if (a.m.output == RVK_DRR_RGBA8_REF) {
     // V/...DRR...cuh:163-166
     const float tw = (float)(a.tint >> 24);
     const float q = tw / (tw + v);
@@ -529,13 +401,12 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
   a.g = vol_geom(vol);
   a.v = view_geom(view);
   a.step_len = min_spacing(vol) * march->step_scale;
-  a.jitter = march->jitter;
-  a.skip = march->skip_rule;
-  a.ref_term = drr->ref_termination;
-  a.intercept = drr->ct_intercept;
-  a.slope = drr->ct_slope;
-  a.offset = drr->ct_offset;
-  a.output = drr->output;
+  a.m.jitter = march->jitter;
+  a.m.ref_term = drr->ref_termination;
+  a.m.intercept = drr->ct_intercept;
+  a.m.slope = drr->ct_slope;
+  a.m.offset = drr->ct_offset;
+  a.m.output = drr->output;
   a.tint = (uint32_t)drr->tint[0] | ((uint32_t)drr->tint[1] << 8) | ((uint32_t)drr->tint[2] << 16) |
            ((uint32_t)drr->tint[3] << 24);
   a.vol = vol->frames[frame];
@@ -543,9 +414,9 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
   const int nblk = launch_grid(view, a.tiles_x);
   return dispatch_vt(vol->dtype, [&](auto tag) {
     using VT = std::remove_pointer_t<decltype(tag)>;
-    const bool pr = march->precise != 0;
-    if (a.skip && pr) hipLaunchKernelGGL((drr_kernel<VT, true, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
-    else if (a.skip) hipLaunchKernelGGL((drr_kernel<VT, true, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    const bool pr = march->precise != 0, skip = march->skip_rule != 0;
+    if (skip && pr) hipLaunchKernelGGL((drr_kernel<VT, true, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else if (skip) hipLaunchKernelGGL((drr_kernel<VT, true, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
     else if (pr) hipLaunchKernelGGL((drr_kernel<VT, false, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
     else hipLaunchKernelGGL((drr_kernel<VT, false, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
     RVK_HIP(hipGetLastError());

@@ -112,6 +112,46 @@ __device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGe
   return c0 + az * (c1 - c0);
 }
 
+// Gradient of the trilinear interpolant above with respect to p (voxel index
+// units), on the same clamped cell: d/dx = lerp_z(lerp_y(v100-v000, v110-v010),
+// lerp_y(v101-v001, v111-v011)), d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0.
+// Axes whose two corners clamp to one voxel get a zero derivative.
+template <typename VT>
+__device__ __forceinline__ V3<float> trilinear_grad(const VT* __restrict__ d, const VolGeom& g, float px, float py,
+                                                    float pz) {
+  float x = px - 0.5f, y = py - 0.5f, z = pz - 0.5f;
+  x = fminf(fmaxf(x, -1.0f), (float)g.n[0]);
+  y = fminf(fmaxf(y, -1.0f), (float)g.n[1]);
+  z = fminf(fmaxf(z, -1.0f), (float)g.n[2]);
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const float ax = x - fx, ay = y - fy, az = z - fz;
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  const int x0 = min(max(ix, 0), g.n[0] - 1), x1 = min(max(ix + 1, 0), g.n[0] - 1);
+  const int y0 = min(max(iy, 0), g.n[1] - 1), y1 = min(max(iy + 1, 0), g.n[1] - 1);
+  const int z0 = min(max(iz, 0), g.n[2] - 1), z1 = min(max(iz + 1, 0), g.n[2] - 1);
+  const int ox0 = brick_x(x0), ox1 = brick_x(x1);
+  const int oy0 = brick_y(y0, g.bys), oy1 = brick_y(y1, g.bys);
+  const int oz0 = brick_z(z0, g.bzs), oz1 = brick_z(z1, g.bzs);
+  const float v000 = ldv(d, ox0 + oy0 + oz0), v100 = ldv(d, ox1 + oy0 + oz0);
+  const float v010 = ldv(d, ox0 + oy1 + oz0), v110 = ldv(d, ox1 + oy1 + oz0);
+  const float v001 = ldv(d, ox0 + oy0 + oz1), v101 = ldv(d, ox1 + oy0 + oz1);
+  const float v011 = ldv(d, ox0 + oy1 + oz1), v111 = ldv(d, ox1 + oy1 + oz1);
+  const float c00 = v000 + ax * (v100 - v000);
+  const float c10 = v010 + ax * (v110 - v010);
+  const float c01 = v001 + ax * (v101 - v001);
+  const float c11 = v011 + ax * (v111 - v011);
+  const float c0 = c00 + ay * (c10 - c00);
+  const float c1 = c01 + ay * (c11 - c01);
+  const float dx00 = v100 - v000, dx10 = v110 - v010, dx01 = v101 - v001, dx11 = v111 - v011;
+  const float dx0 = dx00 + ay * (dx10 - dx00), dx1 = dx01 + ay * (dx11 - dx01);
+  const float dy0 = c10 - c00, dy1 = c11 - c01;
+  V3<float> gr;
+  gr.x = (x1 != x0) ? dx0 + az * (dx1 - dx0) : 0.0f;
+  gr.y = (y1 != y0) ? dy0 + az * (dy1 - dy0) : 0.0f;
+  gr.z = (z1 != z0) ? c1 - c0 : 0.0f;
+  return gr;
+}
+
 // tex1D(normalised, linear, clamp) on an n-entry table.
 __device__ __forceinline__ float tf1d_lookup(const float* t, int n, float u) {
   float x = u * (float)n - 0.5f;
@@ -279,6 +319,120 @@ __device__ __forceinline__ Ray form_ray(const VolGeom& g, const ViewGeom& v, flo
   r.start = s;
   keyhole_values(v, s, r.inc, r.num_steps, r.exclude_start, r.exclude_end);
   return r;
+}
+
+// ---------------------------------------------------------------------------
+// Forward-mode tangent of the ray setup: given dM (the derivative of the
+// baked matrix along one pose direction), the derivatives of the clipped
+// start point and of the step vector.  Values follow form_ray operation for
+// operation, so every branch (clip face, reject) is the one the forward took.
+struct Dl {
+  float v, d;
+};
+__device__ __forceinline__ Dl operator+(Dl a, Dl b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dl operator-(Dl a, Dl b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dl operator*(Dl a, Dl b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dl operator*(float a, Dl b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dl operator/(Dl a, Dl b) {
+  const float q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dl dconst(float c) { return {c, 0.0f}; }
+
+struct RayTangent {
+  V3<float> dstart, dinc;
+};
+
+__device__ __forceinline__ RayTangent form_ray_tangent(const VolGeom& g, const ViewGeom& v, const float* dm,
+                                                       float step_len, int x, int y) {
+  const float vx = v.flipped ? (float)x / (float)v.W : 1.0f - (float)x / (float)v.W;
+  const float vy = (float)y / (float)v.H;
+  float end_depth = 1.0f;
+  if (v.zbuf) {
+    const float u = 1.0f - vx;
+    int zi = (int)floorf(fminf(fmaxf(u * (float)v.zbw, -1.0f), (float)v.zbw));
+    int zj = (int)floorf(fminf(fmaxf(vy * (float)v.zbh, -1.0f), (float)v.zbh));
+    zi = min(max(zi, 0), v.zbw - 1);
+    zj = min(max(zj, 0), v.zbh - 1);
+    end_depth = v.zbuf[(long)zj * v.zbw + zi];
+  }
+  Dl M[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) M[i] = Dl{v.m[i], dm[i]};
+  V3<Dl> s, e, f;
+  s.x = vx * M[0] + vy * M[1] + M[3];
+  s.y = vx * M[4] + vy * M[5] + M[7];
+  s.z = vx * M[8] + vy * M[9] + M[11];
+  const Dl sn = vx * M[12] + vy * M[13] + M[15];
+  e.x = s.x + end_depth * M[2];
+  e.y = s.y + end_depth * M[6];
+  e.z = s.z + end_depth * M[10];
+  const Dl en = sn + end_depth * M[14];
+  f.x = s.x + M[2];
+  f.y = s.y + M[6];
+  f.z = s.z + M[10];
+  const Dl fn = sn + M[14];
+  s.x = s.x / sn; s.y = s.y / sn; s.z = s.z / sn;
+  e.x = e.x / en; e.y = e.y / en; e.z = e.z / en;
+  f.x = f.x / fn; f.y = f.y / fn; f.z = f.z / fn;
+  V3<Dl> d{f.x - s.x, f.y - s.y, f.z - s.z};
+  // clipping planes (constant planes)
+  if (v.n_clip) {
+    int flag = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      if (i >= v.n_clip) break;
+      d.x = e.x - s.x; d.y = e.y - s.y; d.z = e.z - s.z;
+      const float px = v.clip[4 * i], py = v.clip[4 * i + 1], pz = v.clip[4 * i + 2], pw = v.clip[4 * i + 3];
+      const Dl dp = px * d.x + py * d.y + pz * d.z;
+      const Dl num = px * s.x + py * s.y + pz * s.z + dconst(pw);
+      const Dl t = Dl{-num.v, -num.d} / dp;
+      const Dl p0 = s.x + t * d.x, p1 = s.y + t * d.y, p2 = s.z + t * d.z;
+      if (t.v > 0.0f && t.v < 1.0f) {
+        if (dp.v > 0.0f) { s.x = p0; s.y = p1; s.z = p2; }
+        else { e.x = p0; e.y = p1; e.z = p2; }
+      }
+      flag |= (dp.v > 0.0f && t.v > 1.0f);
+      flag |= (dp.v < 0.0f && t.v < 0.0f);
+    }
+    if (flag) s = e;
+  }
+  // volume box
+  d.x = e.x - s.x; d.y = e.y - s.y; d.z = e.z - s.z;
+  const float b0 = g.bounds[0] + 1.0f, b1 = g.bounds[1] - 1.0f;
+  const float b2 = g.bounds[2] + 1.0f, b3 = g.bounds[3] - 1.0f;
+  const float b4 = g.bounds[4] + 1.0f, b5 = g.bounds[5] - 1.0f;
+  auto crop = [&](Dl ds, Dl de) {
+    if (isfinite(ds.v)) {
+      s.x = s.x + d.x * ds; s.y = s.y + d.y * ds; s.z = s.z + d.z * ds;
+      e.x = e.x + d.x * de; e.y = e.y + d.y * de; e.z = e.z + d.z * de;
+    }
+  };
+  Dl ds, de;
+  if (d.x.v > 0.0f) { ds = s.x.v < b0 ? dconst(b0) - s.x : dconst(0.0f); de = e.x.v > b1 ? dconst(b1) - e.x : dconst(0.0f); }
+  else { ds = s.x.v > b1 ? dconst(b1) - s.x : dconst(0.0f); de = e.x.v < b0 ? dconst(b0) - e.x : dconst(0.0f); }
+  crop(ds / d.x, de / d.x);
+  if (d.y.v > 0.0f) { ds = s.y.v < b2 ? dconst(b2) - s.y : dconst(0.0f); de = e.y.v > b3 ? dconst(b3) - e.y : dconst(0.0f); }
+  else { ds = s.y.v > b3 ? dconst(b3) - s.y : dconst(0.0f); de = e.y.v < b2 ? dconst(b2) - e.y : dconst(0.0f); }
+  crop(ds / d.y, de / d.y);
+  if (d.z.v > 0.0f) { ds = s.z.v < b4 ? dconst(b4) - s.z : dconst(0.0f); de = e.z.v > b5 ? dconst(b5) - e.z : dconst(0.0f); }
+  else { ds = s.z.v > b5 ? dconst(b5) - s.z : dconst(0.0f); de = e.z.v < b4 ? dconst(b4) - e.z : dconst(0.0f); }
+  crop(ds / d.z, de / d.z);
+  if (e.x.v > b1 + 1.0f || e.y.v > b3 + 1.0f || e.z.v > b5 + 1.0f || e.x.v < b0 - 1.0f || e.y.v < b2 - 1.0f ||
+      e.z.v < b4 - 1.0f || s.x.v > b1 + 1.0f || s.y.v > b3 + 1.0f || s.z.v > b5 + 1.0f || s.x.v < b0 - 1.0f ||
+      s.y.v < b2 - 1.0f || s.z.v < b4 - 1.0f) {
+    s = e;
+  }
+  d.x = e.x - s.x; d.y = e.y - s.y; d.z = e.z - s.z;
+  const Dl q = d.x * d.x * dconst(g.sp[0]) * dconst(g.sp[0]) + d.y * d.y * dconst(g.sp[1]) * dconst(g.sp[1]) +
+               d.z * d.z * dconst(g.sp[2]) * dconst(g.sp[2]);
+  const float rq = sqrt_ru(q.v);
+  const Dl ns{rq / step_len, (rq > 0.0f ? q.d / (2.0f * rq) : 0.0f) / step_len};
+  RayTangent t;
+  t.dstart = V3<float>{s.x.d, s.y.d, s.z.d};
+  const Dl ix = d.x / ns, iy = d.y / ns, iz = d.z / ns;
+  t.dinc = V3<float>{ix.d, iy.d, iz.d};
+  return t;
 }
 
 // ---------------------------------------------------------------------------

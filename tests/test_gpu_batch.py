@@ -1,0 +1,247 @@
+"""Batched DRR, hipGraph replay, NCC, and the DRR adjoint (GPU).
+
+Gradients are checked against FP64 central finite differences of the CPU
+oracle (h = 1e-3 rad / mm for poses), with the pose matrices formed in double
+precision (no float32 rounding inside the difference quotient).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from robartsvtk_amd import abi, configs
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def baked64(cam, pose, center, W, H, spacing):
+    u, _ = abi.pose_matrix(pose, center)
+    m, _ = abi.compute_matrices(cam, W / H, spacing=spacing, user_matrix=u)
+    return np.array([[m[r, 0] * -2.0, m[r, 1] * 2.0, m[r, 2], m[r, 3] + m[r, 0] - m[r, 1]] for r in range(4)])
+
+
+def ncc64(img, tgt):
+    a = img.astype(np.float64).ravel() - img.mean()
+    b = tgt.astype(np.float64).ravel() - tgt.mean()
+    return float(a @ b / np.sqrt((a @ a) * (b @ b)))
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("schedule", [abi.SCHED_GRID, abi.SCHED_PERSISTENT])
+def test_batch_matches_single_view_renders(gpu_ctx, schedule):
+    torch = _torch()
+    s = configs.scene_c4(scale=0.125, n_poses=24)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    mp, dp = configs.march_params(s.march), configs.drr_params(s.drr)
+    plan = abi.DrrPlan(gpu_ctx, vol, s.views, mp, dp)
+    plan.set_schedule(schedule, 2)
+    out = torch.full((plan.total,), float("nan"), dtype=torch.float32, device="cuda")
+    plan.render(out)
+    torch.cuda.synchronize()
+    for v, view in enumerate(s.views):
+        single = torch.empty((view.height, view.width), dtype=torch.float32, device="cuda")
+        abi.render_drr(gpu_ctx, vol, view, mp, dp, single)
+        torch.cuda.synchronize()
+        assert torch.equal(plan.split(out)[v], single), v
+    # and against the oracle for two views of different size
+    for v in (0, 5):
+        o = orc.drr(s.volume, s.spacing, configs.view_dict(s.views[v]), s.march, s.drr, "f32")
+        assert H.rel_err(plan.split(out)[v].cpu().numpy(), o).max() <= 2e-6
+    plan.close()
+
+
+def test_autotune_and_graph_replays(gpu_ctx):
+    torch = _torch()
+    s = configs.scene_c4(scale=0.125, n_poses=16)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    mp, dp = configs.march_params(s.march), configs.drr_params(s.drr)
+    plan = abi.DrrPlan(gpu_ctx, vol, s.views, mp, dp)
+    out = torch.zeros(plan.total, dtype=torch.float32, device="cuda")
+    ms = plan.autotune(out, reps=2)
+    assert ms > 0 and plan.schedule()[0] in (abi.SCHED_GRID, abi.SCHED_PERSISTENT)
+    plan.render(out)
+    torch.cuda.synchronize()
+    eager = out.clone()
+    plan.capture(out)
+    for rep in range(4):
+        out.fill_(-1.0)                      # eager work between replays
+        torch.cuda.synchronize()
+        plan.render(out)                     # graph replay
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), rep
+    # a persistent-schedule graph needs its queue reset on every replay
+    plan.set_schedule(abi.SCHED_PERSISTENT, 4)
+    plan.capture(out)
+    for rep in range(3):
+        out.zero_()
+        plan.render(out)
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), rep
+    plan.close()
+
+
+# ---------------------------------------------------------------------------
+def _registration_scene(n_views=3, scale=0.125):
+    s = configs.scene_c5(scale=scale, n_views=n_views)
+    c = s.extra["center"]
+    img = s.extra["img"]
+    # targets: FP64 oracle DRRs at the ground-truth poses plus seeded noise
+    rng = np.random.default_rng(5)
+    tg = []
+    for v in range(n_views):
+        vd = {"m": np.zeros(16, np.float32), "W": img, "H": img,
+              "m64": baked64(s.extra["cameras"][v], s.extra["gt_poses"][v], c, img, img, s.spacing)}
+        t = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f64")
+        tg.append((t + rng.normal(0.0, s.extra["noise_sigma"], t.shape)).astype(np.float32))
+    return s, tg
+
+
+def _fd_pose_gradient(s, tg, v, h=1e-3):
+    cam, c, img = s.extra["cameras"][v], s.extra["center"], s.extra["img"]
+    g = np.zeros(6)
+    for i in range(6):
+        vals = []
+        for sgn in (1.0, -1.0):
+            pose = np.array(s.extra["poses"][v], float)
+            pose[i] += sgn * h
+            vd = {"m": np.zeros(16, np.float32), "W": img, "H": img,
+                  "m64": baked64(cam, pose, c, img, img, s.spacing)}
+            I = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f64")
+            vals.append(1.0 - ncc64(I, tg[v]))
+        g[i] = (vals[0] - vals[1]) / (2 * h)
+    return g
+
+
+def _plan_with_derivatives(ctx, s, vol):
+    c, img = s.extra["center"], s.extra["img"]
+    views, dms = [], []
+    for v, cam in enumerate(s.extra["cameras"]):
+        view, dm = abi.posed_view(cam, s.extra["poses"][v], c, img, img, spacing=s.spacing, with_derivatives=True)
+        views.append(view)
+        dms.append(dm)
+    return abi.DrrPlan(ctx, vol, views, configs.march_params(s.march), configs.drr_params(s.drr),
+                       dview=np.stack(dms))
+
+
+def test_register_loss_and_pose_gradient_vs_fp64_fd(gpu_ctx):
+    torch = _torch()
+    s, tg = _registration_scene(n_views=3)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    plan = _plan_with_derivatives(gpu_ctx, s, vol)
+    tgt = torch.from_numpy(np.concatenate([t.ravel() for t in tg])).cuda()
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    loss = torch.empty(plan.n_views, dtype=torch.float32, device="cuda")
+    grad = torch.empty(plan.n_views * 6, dtype=torch.float32, device="cuda")
+    plan.register(out, tgt, loss, grad)
+    torch.cuda.synchronize()
+    loss, grad = loss.cpu().numpy(), grad.cpu().numpy().reshape(-1, 6)
+    img = s.extra["img"]
+    for v in range(plan.n_views):
+        vd = {"m": np.zeros(16, np.float32), "W": img, "H": img,
+              "m64": baked64(s.extra["cameras"][v], s.extra["poses"][v], s.extra["center"], img, img, s.spacing)}
+        I = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f64")
+        l64 = 1.0 - ncc64(I, tg[v])
+        assert abs(loss[v] - l64) <= 1e-4 * max(1.0, abs(l64)), (v, loss[v], l64)
+        fd = _fd_pose_gradient(s, tg, v)
+        err = np.linalg.norm(grad[v] - fd) / np.linalg.norm(fd)
+        print(f"view {v}: loss {loss[v]:.6f} (fp64 {l64:.6f}); grad {grad[v]}; fd {fd}; rel err {err:.2e}")
+        assert err <= 1e-3, (v, err)
+    plan.close()
+
+
+def test_backward_with_explicit_image_gradient_matches_register(gpu_ctx):
+    torch = _torch()
+    s, tg = _registration_scene(n_views=2)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    plan = _plan_with_derivatives(gpu_ctx, s, vol)
+    tgt = torch.from_numpy(np.concatenate([t.ravel() for t in tg])).cuda()
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    loss = torch.empty(plan.n_views, dtype=torch.float32, device="cuda")
+    grad = torch.empty(plan.n_views * 6, dtype=torch.float32, device="cuda")
+    plan.register(out, tgt, loss, grad)
+    loss2 = torch.empty_like(loss)
+    dldi = torch.empty_like(out)
+    plan.ncc(out, tgt, loss2, dldi)
+    grad2 = torch.empty_like(grad)
+    plan.backward(out, dldi, grad_pose=grad2)
+    torch.cuda.synchronize()
+    assert torch.equal(loss, loss2)
+    torch.testing.assert_close(grad2, grad, rtol=1e-5, atol=1e-7)
+    # dloss/dI against the closed form in FP64
+    I = out.cpu().numpy().astype(np.float64)
+    for v in range(plan.n_views):
+        a = I[plan.offsets[v]:plan.offsets[v + 1]]
+        b = tg[v].ravel().astype(np.float64)
+        ac, bc = a - a.mean(), b - b.mean()
+        saa, sbb = ac @ ac, bc @ bc
+        ncc = (ac @ bc) / np.sqrt(saa * sbb)
+        g = -(bc / np.sqrt(saa * sbb) - ncc * ac / saa)
+        np.testing.assert_allclose(dldi.cpu().numpy()[plan.offsets[v]:plan.offsets[v + 1]], g, rtol=1e-4,
+                                   atol=1e-6 * np.abs(g).max())
+    plan.close()
+
+
+def test_registration_graph_replay(gpu_ctx):
+    torch = _torch()
+    s, tg = _registration_scene(n_views=2)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    plan = _plan_with_derivatives(gpu_ctx, s, vol)
+    tgt = torch.from_numpy(np.concatenate([t.ravel() for t in tg])).cuda()
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    loss = torch.empty(plan.n_views, dtype=torch.float32, device="cuda")
+    grad = torch.empty(plan.n_views * 6, dtype=torch.float32, device="cuda")
+    plan.register(out, tgt, loss, grad)
+    torch.cuda.synchronize()
+    ref = (out.clone(), loss.clone(), grad.clone())
+    plan.capture(out, tgt, loss, grad)
+    for rep in range(3):
+        for t in (out, loss, grad):
+            t.fill_(123.0)
+        torch.cuda.synchronize()
+        plan.register(out, tgt, loss, grad)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref[0]) and torch.equal(loss, ref[1]) and torch.equal(grad, ref[2]), rep
+    plan.close()
+
+
+def test_volume_gradient_vs_fp64_fd(gpu_ctx):
+    torch = _torch()
+    s, tg = _registration_scene(n_views=2, scale=0.1)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    plan = _plan_with_derivatives(gpu_ctx, s, vol)
+    tgt = torch.from_numpy(np.concatenate([t.ravel() for t in tg])).cuda()
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    loss = torch.empty(plan.n_views, dtype=torch.float32, device="cuda")
+    dldi = torch.empty_like(out)
+    plan.render(out)
+    plan.ncc(out, tgt, loss, dldi)
+    gvol = torch.zeros(s.volume.size, dtype=torch.float32, device="cuda")
+    plan.backward(out, dldi, grad_volume=gvol)
+    torch.cuda.synchronize()
+    g = gvol.cpu().numpy().reshape(s.volume.shape)
+    img = s.extra["img"]
+    vds = [{"m": np.zeros(16, np.float32), "W": img, "H": img,
+            "m64": baked64(s.extra["cameras"][v], s.extra["poses"][v], s.extra["center"], img, img, s.spacing)}
+           for v in range(plan.n_views)]
+
+    def total_loss(mu):
+        return sum(1.0 - ncc64(orc.drr(mu, s.spacing, vds[v], s.march, s.drr, "f64"), tg[v])
+                   for v in range(plan.n_views))
+
+    # a random direction over the body voxels (mu > 0 in the whole trilinear neighbourhood)
+    rng = np.random.default_rng(9)
+    body = s.volume > 0.015
+    r = np.where(body, rng.standard_normal(s.volume.shape), 0.0).astype(np.float32)
+    eps = 1e-3
+    fd = (total_loss((s.volume + eps * r).astype(np.float32)) -
+          total_loss((s.volume - eps * r).astype(np.float32))) / (2 * eps)
+    an = float((g.astype(np.float64) * r).sum())
+    print(f"directional derivative: analytic {an:.6e}, fd {fd:.6e}")
+    assert abs(an - fd) <= 1e-3 * abs(fd)
+    plan.close()

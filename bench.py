@@ -72,6 +72,81 @@ def bench_drr(scene, steps: int, warmup: int):
             "image_mean": float(img.mean())}
 
 
+def _time(fn, reps: int) -> float:
+    """Mean milliseconds of fn() over reps runs (after one warm-up), HIP events
+    on torch's current stream (the stream the library's context uses)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s = torch.cuda.current_stream()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def bench_batched(scene, reps: int = 5):
+    """Batched forward and forward+backward (registration step) against the
+    plain per-pose loop: one single-view launch per pose for the forward, and
+    13 single-view renders per pose (1 + 12 central differences of the 6-DoF
+    pose) for the gradient.  The plain loop's loss evaluations are left out,
+    which favours the plain loop."""
+    import torch
+
+    from robartsvtk_amd import abi, configs
+
+    ctx = abi.Context(0)
+    vol = abi.Volume.from_array(ctx, scene.volume, scene.spacing)
+    mp, dp = configs.march_params(scene.march), configs.drr_params(scene.drr)
+    views = scene.views
+    n = len(views)
+    plan = abi.DrrPlan(ctx, vol, views, mp, dp, dview=scene.extra["dview"])
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    singles = [torch.empty((v.height, v.width), dtype=torch.float32, device="cuda") for v in views]
+    n_samples = sum(abi.count_ray_steps(ctx, vol, v, mp) for v in views)
+
+    def plain_fwd():
+        for v, o in zip(views, singles):
+            abi.render_drr(ctx, vol, v, mp, dp, o)
+
+    def plain_fd():
+        for v, o in zip(views, singles):
+            for _ in range(13):
+                abi.render_drr(ctx, vol, v, mp, dp, o)
+
+    t_plain = _time(plain_fwd, reps)
+    t_plain_fd = _time(plain_fd, max(1, reps // 2))
+    tuned_ms = plan.autotune(out, reps=3)
+    sched = plan.schedule()
+    plan.capture(out)
+    t_batch = _time(lambda: plan.render(out), reps)
+    plan.render(out)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    tgt = out + 0.01 * torch.randn(out.shape, device="cuda", generator=g)
+    loss = torch.empty(n, dtype=torch.float32, device="cuda")
+    grad = torch.empty(6 * n, dtype=torch.float32, device="cuda")
+    plan.capture(out, tgt, loss, grad)
+    t_reg = _time(lambda: plan.register(out, tgt, loss, grad), reps)
+    torch.cuda.synchronize()
+    assert torch.isfinite(loss).all() and torch.isfinite(grad).all()
+    plan.close()
+    vol.close()
+    ctx.close()
+    return {"poses": n, "pixels": int(sum(v.width * v.height for v in views)), "n_samples": n_samples,
+            "plain_fwd_ms": round(t_plain, 3), "batched_fwd_ms": round(t_batch, 3),
+            "fwd_speedup": round(t_plain / t_batch, 3),
+            "batched_fwd_G_samples_per_s": round(n_samples / (t_batch / 1e3) / 1e9, 2),
+            "schedule": {"kind": "persistent" if sched[0] == abi.SCHED_PERSISTENT else "grid",
+                         "waves_per_simd": sched[1], "autotune_ms": round(tuned_ms, 3)},
+            "plain_fwd_fd_ms": round(t_plain_fd, 3), "batched_fwd_bwd_ms": round(t_reg, 3),
+            "plain_fd_poses_per_s": round(n / (t_plain_fd / 1e3), 2),
+            "batched_fwd_bwd_poses_per_s": round(n / (t_reg / 1e3), 2),
+            "fwd_bwd_speedup": round(t_plain_fd / t_reg, 3)}
+
+
 def cpu_baseline(scene, n_samples: int, runs: int = 3):
     from oracle import oracle as orc
     from robartsvtk_amd import configs
@@ -96,6 +171,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the c4/c5 batching measurements")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -145,6 +221,13 @@ def main() -> int:
                      "kernel": "drr_kernel<short,true,true>", "kernel_ms": round(r["kernel_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes},
     }
+    if not args.no_batched:
+        log("timing c4: 64 poses, uneven detectors, batched vs per-pose")
+        line["batched"] = {"c4": bench_batched(configs.scene_c4(scale=1.0, with_derivatives=True))}
+        log("timing c5: 16-view registration step")
+        line["batched"]["c5"] = bench_batched(configs.scene_c5(scale=1.0))
+        line["batched"]["note"] = ("fwd_bwd = one replayed hipGraph of batched render + NCC + pose adjoint; plain = "
+                                   "13 single-view renders per pose (1 + 12 central differences)")
     if not args.no_cpu_baseline:
         log("timing CPU oracle baseline")
         line["cpu_baseline"] = cpu_baseline(scene, r["n_samples"])

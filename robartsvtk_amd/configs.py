@@ -224,7 +224,7 @@ def random_poses(seed: int, n: int, rot_deg: float, trans_mm: float) -> np.ndarr
     return poses
 
 
-def scene_c4(scale: float = 1.0, seed: int = 4, n_poses: int = 64) -> Scene:
+def scene_c4(scale: float = 1.0, seed: int = 4, n_poses: int = 64, with_derivatives: bool = False) -> Scene:
     dims = (max(8, int(round(512 * scale))), max(8, int(round(512 * scale))), max(8, int(round(400 * scale))))
     spacing = (1.0 / scale if scale < 1.0 else 1.0,) * 3
     vol = abi.synth_ct_phantom(dims, spacing, seed, n_bones=6)
@@ -232,17 +232,25 @@ def scene_c4(scale: float = 1.0, seed: int = 4, n_poses: int = 64) -> Scene:
     poses = random_poses(seed + 1000, n_poses, 10.0, 20.0)
     sizes_r = _rng(seed + 2000, n_poses)
     choices = [256, 512, 768, 1024]
-    views, sizes = [], []
+    views, sizes, dms, cams = [], [], [], []
     half = 0.5 * math.sqrt(sum((dims[a] * spacing[a]) ** 2 for a in range(3))) + 4
     for i in range(n_poses):
         s = max(4, int(round(choices[min(3, int(sizes_r[i] * 4))] * scale)))
         cam = drr_camera(c, 0.0, s, half_extent=half)
-        views.append(abi.posed_view(cam, poses[i], c, s, s, spacing=spacing))
+        if with_derivatives:
+            v, dm = abi.posed_view(cam, poses[i], c, s, s, spacing=spacing, with_derivatives=True)
+            views.append(v)
+            dms.append(dm)
+        else:
+            views.append(abi.posed_view(cam, poses[i], c, s, s, spacing=spacing))
         sizes.append(s)
+        cams.append(cam)
     march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": 1, "precise": 1}
     drr = {"intercept": 0.02, "slope": 2e-5, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L, "ref_termination": 0,
            "tint": (0, 0, 0, 0)}
-    return Scene("c4", vol, spacing, views, march, drr=drr, extra={"poses": poses, "sizes": sizes, "center": c})
+    return Scene("c4", vol, spacing, views, march, drr=drr,
+                 extra={"poses": poses, "sizes": sizes, "center": c, "cameras": cams,
+                        "dview": np.stack(dms) if with_derivatives else None})
 
 
 def mu_from_hu(hu: np.ndarray, mu_water: float = 0.02) -> np.ndarray:
@@ -265,10 +273,14 @@ def scene_c5(scale: float = 1.0, seed: int = 5, n_views: int = 16, noise_sigma: 
     march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": 0, "precise": 1}
     drr = {"intercept": 0.0, "slope": 1.0, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L, "ref_termination": 0,
            "tint": (0, 0, 0, 0)}
-    views = [abi.posed_view(cams[i], poses[i], c, img, img, spacing=spacing) for i in range(n_views)]
+    views, dms = [], []
+    for i in range(n_views):
+        v, dm = abi.posed_view(cams[i], poses[i], c, img, img, spacing=spacing, with_derivatives=True)
+        views.append(v)
+        dms.append(dm)
     return Scene("c5", mu, spacing, views, march, drr=drr,
                  extra={"cameras": cams, "gt_poses": gt, "poses": poses, "center": c, "noise_sigma": noise_sigma,
-                        "img": img})
+                        "img": img, "dview": np.stack(dms)})
 
 
 SCENES = {"c1": scene_c1, "c2": scene_c2, "c3": scene_c3, "c4": scene_c4, "c5": scene_c5}

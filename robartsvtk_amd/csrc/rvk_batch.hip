@@ -40,7 +40,22 @@ struct BatchArgs {
   int n_views;
   int n_items;
   float* out;
+  float* jac;             // 6 x total pixels (SoA) d L_p / d pose_i, or NULL
+  long long total;        // pixels in the packed images
   unsigned* queues;       // kNumXcd work-queue heads (persistent schedule)
+};
+
+// Accumulates A = sum ds*slope*grad mu and B = sum (j+t) ds*slope*grad mu over
+// the accumulated samples of one ray (see the pose adjoint below).
+struct PoseVisit {
+  float ncs, j;
+  V3<float>& A;
+  V3<float>& B;
+  __device__ __forceinline__ void operator()(const V3<float>&, float t, float, const V3<float>& gr) const {
+    const float w = j + t;
+    A.x += ncs * gr.x; A.y += ncs * gr.y; A.z += ncs * gr.z;
+    B.x += w * ncs * gr.x; B.y += w * ncs * gr.y; B.z += w * ncs * gr.z;
+  }
 };
 
 __device__ __forceinline__ int find_view(const int* __restrict__ ib, int n, int item) {
@@ -59,7 +74,10 @@ __device__ __forceinline__ void item_pixel(int tiles16_x, int local, int lane, i
   y = (macro / tiles16_x) * 16 + (q >> 1) * 8 + (lane >> 3);
 }
 
-template <typename VT, bool SKIP, bool PRECISE>
+// One pixel of one view.  With JAC the march also gathers trilinear gradients
+// and the pixel's pose Jacobian dL/dpose_i = A . dstart_i + B . dinc_i is
+// written next to the image (the registration step's single march).
+template <typename VT, bool SKIP, bool PRECISE, bool JAC>
 __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int lane) {
   const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
   const BatchView& bv = a.views[v];
@@ -67,23 +85,43 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
   item_pixel(bv.tiles16_x, item - a.item_begin[v], lane, x, y);
   if (x >= bv.v.W || y >= bv.v.H) return;
   const Ray r = form_ray(a.g, bv.v, a.step_len, x, y);
-  a.out[bv.pix_off + (long long)y * bv.v.W + x] = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a.g, a.m, r, x, y);
+  const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
+  if (!JAC) {
+    a.out[pi] = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a.g, a.m, r, x, y);
+    return;
+  }
+  const float ds = sqrtf(r.inc.x * r.inc.x * a.g.sp[0] * a.g.sp[0] + r.inc.y * r.inc.y * a.g.sp[1] * a.g.sp[1] +
+                         r.inc.z * r.inc.z * a.g.sp[2] * a.g.sp[2]);
+  V3<float> A{0, 0, 0}, B{0, 0, 0};
+  PoseVisit visit{ds * a.m.slope, ray_offset(a.m.jitter, x, y), A, B};
+  a.out[pi] = drr_march<VT, SKIP, true, true>((const VT*)a.vol, a.g, a.m, r, x, y, visit);
+  const bool any = A.x != 0.0f || A.y != 0.0f || A.z != 0.0f || B.x != 0.0f || B.y != 0.0f || B.z != 0.0f;
+#pragma unroll 1
+  for (int i = 0; i < 6; ++i) {
+    float jv = 0.0f;
+    if (any) {
+      const RayTangent tg = form_ray_tangent(a.g, bv.v, bv.dm + 16 * i, a.step_len, x, y);
+      jv = A.x * tg.dstart.x + A.y * tg.dstart.y + A.z * tg.dstart.z + B.x * tg.dinc.x + B.y * tg.dinc.y +
+           B.z * tg.dinc.z;
+    }
+    a.jac[i * a.total + pi] = jv;
+  }
 }
 
 // One wave per tile; workgroups XCD-remapped so neighbouring tiles share an L2.
-template <typename VT, bool SKIP, bool PRECISE>
+template <typename VT, bool SKIP, bool PRECISE, bool JAC>
 __global__ __launch_bounds__(256) void batch_grid_kernel(BatchArgs a) {
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int item = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (item >= a.n_items) return;
-  forward_item<VT, SKIP, PRECISE>(a, item, threadIdx.x & 63);
+  forward_item<VT, SKIP, PRECISE, JAC>(a, item, threadIdx.x & 63);
 }
 
 // Persistent: the tile range is cut into kNumXcd contiguous queues.  A wave
 // drains the queue of the XCD its workgroup was dispatched to (blockIdx mod
 // kNumXcd under round-robin dispatch), then the others in turn; every queue
 // is finite, so every wave reaches the end.
-template <typename VT, bool SKIP, bool PRECISE>
+template <typename VT, bool SKIP, bool PRECISE, bool JAC>
 __global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int home = blockIdx.x % kNumXcd;
@@ -97,7 +135,7 @@ __global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a) {
       w = __builtin_amdgcn_readfirstlane(w);
       const int item = begin + w;
       if (item >= end) break;
-      forward_item<VT, SKIP, PRECISE>(a, item, lane);
+      forward_item<VT, SKIP, PRECISE, JAC>(a, item, lane);
     }
   }
 }
@@ -222,21 +260,6 @@ __device__ __forceinline__ float dloss_at(const float* dldi, const double* stats
   return dldi ? dldi[pi] : dloss_dimage(stats + 8 * v, img[pi], tgt[pi]);
 }
 
-template <typename VT>
-struct PoseVisitT {
-  const VT* vol;
-  const VolGeom& g;
-  float ncs, j;
-  V3<float>& A;
-  V3<float>& B;
-  __device__ __forceinline__ void operator()(const V3<float>& q, float t, float) const {
-    const V3<float> gr = trilinear_grad(vol, g, q.x, q.y, q.z);
-    const float w = j + t;
-    A.x += ncs * gr.x; A.y += ncs * gr.y; A.z += ncs * gr.z;
-    B.x += w * ncs * gr.x; B.y += w * ncs * gr.y; B.z += w * ncs * gr.z;
-  }
-};
-
 template <typename VT, bool SKIP>
 __global__ __launch_bounds__(256) void adjoint_pose_kernel(AdjArgs a) {
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -255,8 +278,8 @@ __global__ __launch_bounds__(256) void adjoint_pose_kernel(AdjArgs a) {
                            r.inc.z * r.inc.z * g.sp[2] * g.sp[2]);
     V3<float> A{0, 0, 0}, B{0, 0, 0};
     const float jt = ray_offset(a.b.m.jitter, x, y);
-    PoseVisitT<VT> visit{(const VT*)a.b.vol, g, ds * a.b.m.slope, jt, A, B};
-    const float I = drr_march<VT, SKIP, true>((const VT*)a.b.vol, g, a.b.m, r, x, y, visit);
+    PoseVisit visit{ds * a.b.m.slope, jt, A, B};
+    const float I = drr_march<VT, SKIP, true, true>((const VT*)a.b.vol, g, a.b.m, r, x, y, visit);
     const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
     const float dLdI = dloss_at(a.dldi, a.stats, a.img, a.tgt, v, pi);
     // dI/dL = -I for exp(-L) output, 1 for L output
@@ -294,6 +317,44 @@ __global__ __launch_bounds__(256) void pose_reduce_kernel(const int* item_begin,
     __syncthreads();
   }
   if (threadIdx.x < 6) grad[6 * v + threadIdx.x] = (float)red[threadIdx.x][0];
+}
+
+// d loss / d pose from the stored per-pixel Jacobians (registration step).
+struct JacGradArgs {
+  const BatchView* views;
+  const int* item_begin;
+  int n_views, n_items;
+  const float* img;
+  const float* tgt;
+  const float* jac;
+  long long total;
+  const double* stats;
+  int output;
+  float* part;
+};
+
+__global__ __launch_bounds__(256) void jac_grad_kernel(JacGradArgs a) {
+  const int item = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= a.n_items) return;
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
+  const BatchView& bv = a.views[v];
+  int x, y;
+  item_pixel(bv.tiles16_x, item - a.item_begin[v], lane, x, y);
+  float c[6] = {0, 0, 0, 0, 0, 0};
+  if (x < bv.v.W && y < bv.v.H) {
+    const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
+    const float I = a.img[pi];
+    const float dLdI = dloss_dimage(a.stats + 8 * v, I, a.tgt[pi]);
+    const float coef = (a.output == RVK_DRR_F32_EXP_NEG_L) ? -dLdI * I : dLdI;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) c[i] = coef * a.jac[i * a.total + pi];
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) c[i] = wave_sumf(c[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) a.part[(long long)item * 6 + i] = c[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -470,6 +531,7 @@ struct rvk_drr_plan {
   double* d_stats = nullptr;  // n_views x 8
   float* d_pose_part = nullptr;  // n_items x 6
   RayRec* d_rays = nullptr;      // per pixel, allocated by the first volume gradient
+  float* d_jac = nullptr;        // 6 x total pixels, allocated with the plan when it has pose derivatives
   int schedule = RVK_SCHED_GRID;
   int waves_per_simd = 4;
   int n_cu = 256;
@@ -544,11 +606,13 @@ BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
   a.n_views = p->n_views;
   a.n_items = p->n_items;
   a.out = out;
+  a.jac = p->d_jac;
+  a.total = p->pix_off[p->n_views];
   a.queues = p->d_queues;
   return a;
 }
 
-int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s) {
+int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s, bool jac = false) {
   BatchArgs a = batch_args(p, out);
   const bool skip = p->march.skip_rule != 0, pr = p->march.precise != 0;
   return dispatch_vt_b(p->vol->dtype, [&](auto tag) {
@@ -556,15 +620,17 @@ int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s) {
     if (p->schedule == RVK_SCHED_PERSISTENT) {
       hipLaunchKernelGGL(reset_queues_kernel, dim3(1), dim3(64), 0, s, p->d_queues);
       const int blocks = std::max(1, std::min(p->n_cu * p->waves_per_simd, (p->n_items + 3) / 4));
-#define RVK_LP(S, P) hipLaunchKernelGGL((batch_persistent_kernel<VT, S, P>), dim3(blocks), dim3(256), 0, s, a)
-      if (skip) { if (pr) RVK_LP(true, true); else RVK_LP(true, false); }
-      else { if (pr) RVK_LP(false, true); else RVK_LP(false, false); }
+#define RVK_LP(S, P, J) hipLaunchKernelGGL((batch_persistent_kernel<VT, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
+      if (jac) { if (skip) RVK_LP(true, true, true); else RVK_LP(false, true, true); }
+      else if (skip) { if (pr) RVK_LP(true, true, false); else RVK_LP(true, false, false); }
+      else { if (pr) RVK_LP(false, true, false); else RVK_LP(false, false, false); }
 #undef RVK_LP
     } else {
       const int blocks = (p->n_items + 3) / 4;
-#define RVK_LG(S, P) hipLaunchKernelGGL((batch_grid_kernel<VT, S, P>), dim3(blocks), dim3(256), 0, s, a)
-      if (skip) { if (pr) RVK_LG(true, true); else RVK_LG(true, false); }
-      else { if (pr) RVK_LG(false, true); else RVK_LG(false, false); }
+#define RVK_LG(S, P, J) hipLaunchKernelGGL((batch_grid_kernel<VT, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
+      if (jac) { if (skip) RVK_LG(true, true, true); else RVK_LG(false, true, true); }
+      else if (skip) { if (pr) RVK_LG(true, true, false); else RVK_LG(true, false, false); }
+      else { if (pr) RVK_LG(false, true, false); else RVK_LG(false, false, false); }
 #undef RVK_LG
     }
     RVK_HIP(hipGetLastError());
@@ -587,27 +653,30 @@ NccArgs ncc_args(const rvk_drr_plan* p, const float* img, const float* tgt, floa
   return n;
 }
 
+// The registration step: one march per pixel producing the image and its
+// pose Jacobian, then NCC (tile moments, per-view statistics) and the
+// Jacobian-weighted reduction to d loss / d pose.
 int launch_register(rvk_drr_plan* p, float* out, const float* tgt, float* loss, float* grad, hipStream_t s) {
-  int rc = launch_forward(p, out, s);
+  if (!p->d_jac) return fail(RVK_ERR_NOT_READY, "the plan was created without pose derivatives (dview)");
+  int rc = launch_forward(p, out, s, true);
   if (rc) return rc;
   const int blocks = (p->n_items + 3) / 4;
   NccArgs n = ncc_args(p, out, tgt, loss, nullptr);
   hipLaunchKernelGGL(ncc_partial_kernel, dim3(blocks), dim3(256), 0, s, n);
   hipLaunchKernelGGL(ncc_view_kernel, dim3(p->n_views), dim3(256), 0, s, n);
-  AdjArgs aa;
-  aa.b = batch_args(p, out);
-  aa.img = out;
-  aa.dldi = nullptr;
-  aa.stats = p->d_stats;
-  aa.tgt = tgt;
-  aa.part = p->d_pose_part;
-  rc = dispatch_vt_b(p->vol->dtype, [&](auto tag) {
-    using VT = std::remove_pointer_t<decltype(tag)>;
-    if (p->march.skip_rule) hipLaunchKernelGGL((adjoint_pose_kernel<VT, true>), dim3(blocks), dim3(256), 0, s, aa);
-    else hipLaunchKernelGGL((adjoint_pose_kernel<VT, false>), dim3(blocks), dim3(256), 0, s, aa);
-    return (int)RVK_OK;
-  });
-  if (rc) return rc;
+  JacGradArgs ja;
+  ja.views = p->d_views;
+  ja.item_begin = p->d_item_begin;
+  ja.n_views = p->n_views;
+  ja.n_items = p->n_items;
+  ja.img = out;
+  ja.tgt = tgt;
+  ja.jac = p->d_jac;
+  ja.total = p->pix_off[p->n_views];
+  ja.stats = p->d_stats;
+  ja.output = p->drr.output;
+  ja.part = p->d_pose_part;
+  hipLaunchKernelGGL(jac_grad_kernel, dim3(blocks), dim3(256), 0, s, ja);
   hipLaunchKernelGGL(pose_reduce_kernel, dim3(p->n_views), dim3(256), 0, s, p->d_item_begin, p->d_pose_part, grad);
   RVK_HIP(hipGetLastError());
   return RVK_OK;
@@ -677,6 +746,7 @@ int rvk_drr_plan_create(rvk_ctx* ctx, const rvk_volume* vol, int frame, int n_vi
   if (e == hipSuccess) e = alloc((void**)&p->d_part, sizeof(double) * 5 * (size_t)p->n_items);
   if (e == hipSuccess) e = alloc((void**)&p->d_stats, sizeof(double) * 8 * (size_t)n_views);
   if (e == hipSuccess) e = alloc((void**)&p->d_pose_part, sizeof(float) * 6 * (size_t)p->n_items);
+  if (e == hipSuccess && dview) e = alloc((void**)&p->d_jac, sizeof(float) * 6 * (size_t)p->pix_off[n_views]);
   if (e == hipSuccess) e = hipMemcpy(p->d_views, hv.data(), sizeof(BatchView) * n_views, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(p->d_item_begin, p->item_begin.data(), sizeof(int) * (n_views + 1), hipMemcpyHostToDevice);
@@ -702,12 +772,14 @@ int rvk_drr_plan_destroy(rvk_drr_plan* p) {
   hipFree(p->d_stats);
   hipFree(p->d_pose_part);
   if (p->d_rays) hipFree(p->d_rays);
+  if (p->d_jac) hipFree(p->d_jac);
   delete p;
   return RVK_OK;
 }
 
 int rvk_drr_plan_update_views(rvk_drr_plan* p, const rvk_view* views, const float* dview) {
   if (!p || !views) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (p->d_jac && !dview) return fail(RVK_ERR_INVALID_ARGUMENT, "this plan carries pose derivatives: pass dview");
   std::vector<BatchView> hv;
   int rc = fill_views(p, views, dview, hv);
   if (rc) return rc;

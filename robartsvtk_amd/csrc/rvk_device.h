@@ -112,13 +112,15 @@ __device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGe
   return c0 + az * (c1 - c0);
 }
 
-// Gradient of the trilinear interpolant above with respect to p (voxel index
-// units), on the same clamped cell: d/dx = lerp_z(lerp_y(v100-v000, v110-v010),
-// lerp_y(v101-v001, v111-v011)), d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0.
-// Axes whose two corners clamp to one voxel get a zero derivative.
+// Trilinear value and its gradient with respect to p (voxel index units) on
+// the same clamped cell.  The value is computed with exactly the operations
+// of trilinear(), so branch decisions made on it match the plain caster.
+// d/dx = lerp_z(lerp_y(v100-v000, v110-v010), lerp_y(v101-v001, v111-v011)),
+// d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0; axes whose two corners
+// clamp to one voxel get a zero derivative.
 template <typename VT>
-__device__ __forceinline__ V3<float> trilinear_grad(const VT* __restrict__ d, const VolGeom& g, float px, float py,
-                                                    float pz) {
+__device__ __forceinline__ float trilinear_vg(const VT* __restrict__ d, const VolGeom& g, float px, float py, float pz,
+                                              V3<float>& gr) {
   float x = px - 0.5f, y = py - 0.5f, z = pz - 0.5f;
   x = fminf(fmaxf(x, -1.0f), (float)g.n[0]);
   y = fminf(fmaxf(y, -1.0f), (float)g.n[1]);
@@ -145,11 +147,10 @@ __device__ __forceinline__ V3<float> trilinear_grad(const VT* __restrict__ d, co
   const float dx00 = v100 - v000, dx10 = v110 - v010, dx01 = v101 - v001, dx11 = v111 - v011;
   const float dx0 = dx00 + ay * (dx10 - dx00), dx1 = dx01 + ay * (dx11 - dx01);
   const float dy0 = c10 - c00, dy1 = c11 - c01;
-  V3<float> gr;
   gr.x = (x1 != x0) ? dx0 + az * (dx1 - dx0) : 0.0f;
   gr.y = (y1 != y0) ? dy0 + az * (dy1 - dy0) : 0.0f;
   gr.z = (z1 != z0) ? c1 - c0 : 0.0f;
-  return gr;
+  return c0 + az * (c1 - c0);
 }
 
 // tex1D(normalised, linear, clamp) on an n-entry table.

@@ -75,14 +75,15 @@ struct DrrMarchArgs {
 };
 
 struct NoVisit {
-  __device__ __forceinline__ void operator()(const V3<float>&, float, float) const {}
+  __device__ __forceinline__ void operator()(const V3<float>&, float, float, const V3<float>&) const {}
 };
 
 // Line integral along one ray.  The accumulator starts at ln(256*offset) in
-// the reference RGBA8 mode and at 0 in the float modes.  `visit(q, t, alpha)`
-// is called for every accumulated sample, at position q = p0 + t*inc in
-// PRECISE mode (the adjoint re-marches through the same control flow).
-template <typename VT, bool SKIP, bool PRECISE, typename F = NoVisit>
+// the reference RGBA8 mode and at 0 in the float modes.  `visit(q, t, alpha,
+// grad)` is called for every accumulated sample, at position q = p0 + t*inc in
+// PRECISE mode; with VG the trilinear gradient at q comes from the same
+// gathers as the value (the adjoint marches through this very control flow).
+template <typename VT, bool SKIP, bool PRECISE, bool VG = false, typename F = NoVisit>
 __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const VolGeom& g, const DrrMarchArgs& a,
                                            const Ray& r, int x, int y, F&& visit = F()) {
   const bool ref_mode = a.output == RVK_DRR_RGBA8_REF;
@@ -112,7 +113,8 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
   while (k > 0) {
     const bool in_key = (es >= k && ee < k);
     const V3<float> q = cur_pos<PRECISE>(p, p0, inc, kst, k);
-    const float alpha = cs * trilinear(vol, g, q.x, q.y, q.z) + ci;
+    V3<float> gr{0.0f, 0.0f, 0.0f};
+    const float alpha = cs * (VG ? trilinear_vg(vol, g, q.x, q.y, q.z, gr) : trilinear(vol, g, q.x, q.y, q.z)) + ci;
     const float t = (float)(kst - k);
     if (SKIP) {
       if (alpha < 0.0f && !in_key) {
@@ -120,7 +122,7 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
         sb = 0;
         if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
         else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
-        if (!sa) { add(alpha); visit(q, t, alpha); }
+        if (!sa) { add(alpha); visit(q, t, alpha, gr); }
         if (a.ref_term && acc <= stop_at) break;
       } else {
         if (!sa) { step(p, inc); k--; }
@@ -134,7 +136,7 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
       k--;
       if (alpha < 0.0f && !in_key) {
         add(alpha);
-        visit(q, t, alpha);
+        visit(q, t, alpha, gr);
         if (a.ref_term && acc <= stop_at) break;
       }
     }

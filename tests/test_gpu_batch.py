@@ -148,10 +148,19 @@ def test_register_loss_and_pose_gradient_vs_fp64_fd(gpu_ctx):
         I = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f64")
         l64 = 1.0 - ncc64(I, tg[v])
         assert abs(loss[v] - l64) <= 1e-4 * max(1.0, abs(l64)), (v, loss[v], l64)
-        fd = _fd_pose_gradient(s, tg, v)
-        err = np.linalg.norm(grad[v] - fd) / np.linalg.norm(fd)
-        print(f"view {v}: loss {loss[v]:.6f} (fp64 {l64:.6f}); grad {grad[v]}; fd {fd}; rel err {err:.2e}")
-        assert err <= 1e-3, (v, err)
+        # h = 1e-3 rad / mm moves samples by ~1% of a voxel under rotation, so
+        # the difference quotient straddles kinks of the (C0) trilinear
+        # interpolant; the quotient converges to the analytic gradient as h -> 0
+        fd3 = _fd_pose_gradient(s, tg, v, 1e-3)
+        fd6 = _fd_pose_gradient(s, tg, v, 1e-6)
+        e3r = np.linalg.norm(grad[v][:3] - fd3[:3]) / np.linalg.norm(fd3[:3])
+        e3t = np.linalg.norm(grad[v][3:] - fd3[3:]) / np.linalg.norm(fd3[3:])
+        e6 = np.linalg.norm(grad[v] - fd6) / np.linalg.norm(fd6)
+        print(f"view {v}: loss {loss[v]:.6f} (fp64 {l64:.6f}); grad {grad[v]}; fd(1e-6) {fd6}; "
+              f"rel err vs fd(1e-3): rot {e3r:.2e} trans {e3t:.2e}; vs fd(1e-6) {e6:.2e}")
+        assert e6 <= 1e-3, (v, e6)
+        assert e3t <= 1e-3, (v, e3t)
+        assert e3r <= 5e-2, (v, e3r)
     plan.close()
 
 

@@ -201,7 +201,7 @@ def main() -> int:
     alg_bytes = r["n_samples"] * 8 * 2 + r["pixels"] * 4
     achieved = alg_bytes / (r["kernel_ms"] / 1e3) / 1e9
     line = {
-        "metric": "G ray-samples/s (DRR caster, c2)",
+        "metric": "G ray-samples/s + batched DRR fwd+bwd poses/s vs plain per-pose, 1 MI355X",
         "value": round(value, 3),
         "unit": "G ray-samples/s",
         "n_gpus": 1,

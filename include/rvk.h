@@ -153,6 +153,14 @@ int rvk_ctx_device(const rvk_ctx* ctx, int* device);
 int rvk_ctx_stream(const rvk_ctx* ctx, void** hip_stream);
 int rvk_ctx_set_stream(rvk_ctx* ctx, void* hip_stream); /* NULL: back to the context's own stream */
 int rvk_ctx_sync(rvk_ctx* ctx);                          /* CallSyncThreads */
+/* Device memory on the context's device (the output and z-buffer handling the
+ * reference's output handler did with raw CUDA calls,
+ * V/vtkCudaOutputImageInformationHandler.cxx:316-399).  Copies are ordered on
+ * the context stream; to-host copies return once the data has arrived. */
+int rvk_device_alloc(rvk_ctx* ctx, size_t bytes, void** out);
+int rvk_device_free(rvk_ctx* ctx, void* ptr);
+int rvk_copy_to_device(rvk_ctx* ctx, void* dst, const void* src, size_t bytes);
+int rvk_copy_to_host(rvk_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- volumes --------------------------------------------------------------
  * Replaces CUDA_castBuffer<T,float> + *_loadImageInfo / changeFrame /

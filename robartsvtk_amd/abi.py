@@ -87,6 +87,10 @@ _SIGS = {
     "rvk_ctx_stream": (_I, [_P, C.POINTER(_P)]),
     "rvk_ctx_set_stream": (_I, [_P, _P]),
     "rvk_ctx_sync": (_I, [_P]),
+    "rvk_device_alloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
+    "rvk_device_free": (_I, [_P, _P]),
+    "rvk_copy_to_device": (_I, [_P, _P, _P, C.c_size_t]),
+    "rvk_copy_to_host": (_I, [_P, _P, _P, C.c_size_t]),
     "rvk_volume_create": (_I, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double), _I, _I, C.POINTER(_P)]),
     "rvk_volume_set_frame": (_I, [_P, _I, _P, _I]),
     "rvk_volume_compute_gradmag": (_I, [_P, _I, C.c_float]),

@@ -60,5 +60,23 @@ def build_library(verbose: bool = False, jobs: int | None = None) -> Path:
     return LIB
 
 
+TESTS = PKG.parent / "tests" / "cpp"
+MAPPER_TEST = TESTS / "rvk_mapper_test"
+
+
+def build_tests(verbose: bool = False) -> Path:
+    """The C++ mapper checks (tests/cpp/mapper_test.cpp), linked against the
+    library with an rpath relative to the executable."""
+    cmd = [HIPCC, *COMMON, "-x", "c++", str(TESTS / "mapper_test.cpp"), "-x", "none", "-o", str(MAPPER_TEST),
+           f"-L{PKG}", "-lrvkhip", "-Wl,-rpath,$ORIGIN/../../robartsvtk_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {MAPPER_TEST}", file=sys.stderr)
+    return MAPPER_TEST
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_tests(verbose=True)

@@ -292,6 +292,40 @@ int rvk_ctx_sync(rvk_ctx* ctx) {
   return RVK_OK;
 }
 
+int rvk_device_alloc(rvk_ctx* ctx, size_t bytes, void** out) {
+  if (!ctx || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (bytes == 0) return RVK_OK;
+  RVK_HIP(hipSetDevice(ctx->device));
+  RVK_HIP(hipMalloc(out, bytes));
+  return RVK_OK;
+}
+
+int rvk_device_free(rvk_ctx* ctx, void* ptr) {
+  if (!ctx) return fail(RVK_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  if (!ptr) return RVK_OK;
+  RVK_HIP(hipSetDevice(ctx->device));
+  RVK_HIP(hipStreamSynchronize(ctx->stream));
+  RVK_HIP(hipFree(ptr));
+  return RVK_OK;
+}
+
+int rvk_copy_to_device(rvk_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  RVK_HIP(hipSetDevice(ctx->device));
+  RVK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  RVK_HIP(hipStreamSynchronize(ctx->stream));
+  return RVK_OK;
+}
+
+int rvk_copy_to_host(rvk_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  RVK_HIP(hipSetDevice(ctx->device));
+  RVK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  RVK_HIP(hipStreamSynchronize(ctx->stream));
+  return RVK_OK;
+}
+
 // ---- volumes ----------------------------------------------------------------
 int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[3], int dtype, int n_frames,
                       rvk_volume** out) {

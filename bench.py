@@ -72,6 +72,69 @@ def bench_drr(scene, steps: int, warmup: int):
             "image_mean": float(img.mean())}
 
 
+def bench_compositing(scene, steps: int, warmup: int):
+    """1D (c1) or 2D (c3) compositing caster.  N_samples is the plain march
+    (skip rule off) up to opacity termination: one render with the skip rule
+    off writes the per-pixel sample count into the steps-taken side output,
+    and its sum is the count.  The timed renders use the configured march
+    (skip rule on)."""
+    import numpy as np
+    import torch
+
+    from robartsvtk_amd import abi, configs
+
+    ctx = abi.Context(0)
+    vol = abi.Volume.from_array(ctx, scene.volume, scene.spacing)
+    view = scene.views[0]
+    W, H = view.width, view.height
+    two_d = scene.tf2d is not None
+    if two_d:
+        tf = scene.tf2d
+        vol.compute_gradmag(0, tf["gradmag_max"])
+        tfh = abi.TF2D(ctx, tf["tables"], configs.tf2d_params(tf["params"]))
+
+        def render(mp, o, aux=None):
+            abi.render_2d(ctx, vol, tfh, view, mp, o, True, aux)
+    else:
+        tf = scene.tf1d
+        tfh = abi.TF1D(ctx, tf["alpha"], np.stack([tf["r"], tf["g"], tf["b"]], 1), configs.tf1d_params(tf["params"]),
+                       galpha=tf["galpha"])
+
+        def render(mp, o, aux=None):
+            abi.render_1d(ctx, vol, tfh, view, mp, o, aux)
+    out = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
+    steps_taken = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+    plain = configs.march_params(dict(scene.march, skip_rule=0))
+    render(plain, out, abi.AuxOut(steps_taken.data_ptr(), None, None))
+    torch.cuda.synchronize()
+    n_samples = int(steps_taken.double().sum().item())
+    mp = configs.march_params(scene.march)
+    for _ in range(warmup):
+        render(mp, out)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        render(mp, out)
+        b.record(stream)
+    torch.cuda.synchronize()
+    kern_ms = statistics.fmean(a.elapsed_time(b) for a, b in ev)
+    coverage = float((out[..., 3] > 0).float().mean().item())
+    tfh.close()
+    vol.close()
+    ctx.close()
+    vbytes = scene.volume.dtype.itemsize
+    per_sample = 8 * vbytes + (8 if two_d else 0)  # + the u8 gradient-magnitude footprint
+    alg_bytes = n_samples * per_sample + W * H * 4
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    return {"n_samples": n_samples, "pixels": W * H, "ms_per_render": round(kern_ms, 4),
+            "G_samples_per_s": round(n_samples / (kern_ms / 1e3) / 1e9, 3),
+            "bytes_per_sample": per_sample, "covered_pixel_fraction": round(coverage, 4),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+
+
 def _time(fn, reps: int) -> float:
     """Mean milliseconds of fn() over reps runs (after one warm-up), HIP events
     on torch's current stream (the stream the library's context uses)."""
@@ -172,6 +235,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the c4/c5 batching measurements")
+    ap.add_argument("--no-casters", action="store_true", help="skip the c1/c3 compositing-caster measurements")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -221,6 +285,12 @@ def main() -> int:
                      "kernel": "drr_kernel<short,true,true>", "kernel_ms": round(r["kernel_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes},
     }
+    if not args.no_casters:
+        log("timing c1: 1D TF caster, 128^3 u8, 512x512")
+        line["casters"] = {"c1": bench_compositing(configs.scene_c1(scale=1.0), args.steps, args.warmup)}
+        log("timing c3: 2D TF caster, 512^3 u16 + u8 gradient magnitude, 1920x1080")
+        line["casters"]["c3"] = bench_compositing(configs.scene_c3(scale=1.0), max(5, args.steps // 5),
+                                                  min(args.warmup, 2))
     if not args.no_batched:
         log("timing c4: 64 poses, uneven detectors, batched vs per-pose")
         line["batched"] = {"c4": bench_batched(configs.scene_c4(scale=1.0, with_derivatives=True))}

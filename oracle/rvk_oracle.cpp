@@ -9,7 +9,8 @@
  * text.  Texture hardware is restated from its documented behaviour:
  *   - 3D volume: unnormalised coordinates, linear filter, clamp.  A sample at
  *     p reads the trilinear value at p - 0.5 (texel centres at i + 0.5), with
- *     exact (not 9-bit fixed-point) weights.
+ *     exact (not 9-bit fixed-point) weights; each linear interpolation is
+ *     a + t (b - a) with one rounding (fma), as the GPU casters evaluate it.
  *   - 1D TF: normalised coordinates, linear filter, clamp: x = u*N - 0.5.
  *   - 2D TF and z-buffer: normalised coordinates, point filter, clamp.
  * Directed-rounding square roots (__fsqrt_ru/__fsqrt_rz) are emulated exactly
@@ -127,12 +128,16 @@ template <typename VT> struct Grid {
   int nx, ny, nz;
 };
 
-template <typename T, typename VT> inline T trilinear(const Grid<VT>& g, T px, T py, T pz) {
-  T x = px - T(0.5), y = py - T(0.5), z = pz - T(0.5);
+// a + t (b - a) with one rounding of the product-sum
+template <typename T> inline T lerp(T a, T b, T t) { return std::fma(t, b - a, a); }
+
+// Trilinear value at the half-texel-shifted coordinate h = p - 0.5 (the
+// marches carry h; sampling at p means passing p - 0.5).
+template <typename T, typename VT> inline T trilinear(const Grid<VT>& g, T hx, T hy, T hz) {
   // clamping the coordinate to [-1, n] leaves clamp-addressed results unchanged
-  x = std::fmin(std::fmax(x, T(-1)), (T)g.nx);
-  y = std::fmin(std::fmax(y, T(-1)), (T)g.ny);
-  z = std::fmin(std::fmax(z, T(-1)), (T)g.nz);
+  T x = std::fmin(std::fmax(hx, T(-1)), (T)g.nx);
+  T y = std::fmin(std::fmax(hy, T(-1)), (T)g.ny);
+  T z = std::fmin(std::fmax(hz, T(-1)), (T)g.nz);
   T fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
   T ax = x - fx, ay = y - fy, az = z - fz;
   int ix = (int)fx, iy = (int)fy, iz = (int)fz;
@@ -143,13 +148,10 @@ template <typename T, typename VT> inline T trilinear(const Grid<VT>& g, T px, T
   auto at = [&](int i, int j, int k) { return (T)g.d[i + j * sy + k * sz]; };
   T v000 = at(x0, y0, z0), v100 = at(x1, y0, z0), v010 = at(x0, y1, z0), v110 = at(x1, y1, z0);
   T v001 = at(x0, y0, z1), v101 = at(x1, y0, z1), v011 = at(x0, y1, z1), v111 = at(x1, y1, z1);
-  T c00 = v000 + ax * (v100 - v000);
-  T c10 = v010 + ax * (v110 - v010);
-  T c01 = v001 + ax * (v101 - v001);
-  T c11 = v011 + ax * (v111 - v011);
-  T c0 = c00 + ay * (c10 - c00);
-  T c1 = c01 + ay * (c11 - c01);
-  return c0 + az * (c1 - c0);
+  T c00 = lerp(v000, v100, ax), c10 = lerp(v010, v110, ax);
+  T c01 = lerp(v001, v101, ax), c11 = lerp(v011, v111, ax);
+  T c0 = lerp(c00, c10, ay), c1 = lerp(c01, c11, ay);
+  return lerp(c0, c1, az);
 }
 
 // 1D TF: tex1D, normalised, linear, clamp (V/...1D...cuh:248-267).
@@ -160,8 +162,7 @@ template <typename T> inline T tf1d(const float* t, int n, T u) {
   T a = x - fx;
   int i = (int)fx;
   int i0 = std::min(std::max(i, 0), n - 1), i1 = std::min(std::max(i + 1, 0), n - 1);
-  T v0 = (T)t[i0], v1 = (T)t[i1];
-  return v0 + a * (v1 - v0);
+  return lerp((T)t[i0], (T)t[i1], a);
 }
 
 // 2D TF: tex2D, normalised, point, clamp (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:21-25).
@@ -407,13 +408,15 @@ T drr_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_march* mp, const or
   int ee = k - fl2i_rd(r.exclude_end);
   ctl->ints(k, es, ee);
   const int kst = k;
-  const V3<T> p0 = p;
-  // precise mode: the position follows from the step index (no drift from
-  // repeated p += inc); reference mode: the incrementally updated p.
+  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+  // Sample coordinates are carried half-texel shifted (h = p - 0.5).
+  // Precise mode: h follows from the step index, (p0 - 0.5) + t inc with one
+  // rounding per axis (no drift from repeated p += inc); reference mode: the
+  // incrementally updated p, minus 0.5.
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return p;
+    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{p0.x + t * inc.x, p0.y + t * inc.y, p0.z + t * inc.z};
+    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
   };
   T comp = T(0);  // Kahan compensation of the line integral (precise mode)
   auto add = [&](T v) {
@@ -428,7 +431,7 @@ T drr_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_march* mp, const or
     const bool in_key = (es >= k && ee < k);
     if (ctl->count) ++*ctl->count;
     const V3<T> q = cur();
-    const T alpha = cs * trilinear<T>(g, q.x, q.y, q.z) + ci;
+    const T alpha = std::fma(cs, trilinear<T>(g, q.x, q.y, q.z), ci);
     if (mp->skip_rule) {
       if (ctl->decide(alpha < T(0) && !in_key)) {
         a = b;
@@ -524,13 +527,15 @@ RGBAT<T> cast1d_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_tf1d* tf,
   ctl->ints(k, es, ee);
   k0 = k;
   const int kst = k;
-  const V3<T> p0 = p;
-  // precise mode: the position follows from the step index (no drift from
-  // repeated p += inc); reference mode: the incrementally updated p.
+  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+  // Sample coordinates are carried half-texel shifted (h = p - 0.5).
+  // Precise mode: h follows from the step index, (p0 - 0.5) + t inc with one
+  // rounding per axis (no drift from repeated p += inc); reference mode: the
+  // incrementally updated p, minus 0.5.
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return p;
+    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{p0.x + t * inc.x, p0.y + t * inc.y, p0.z + t * inc.z};
+    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
   };
   int a = 0, b = 0;
   auto accumulate = [&](T alpha, T u) {
@@ -619,13 +624,15 @@ RGBAT<T> cast2d_ray(const Grid<VT>& g, const Grid<uint8_t>* gm_grid, const VolIn
   ctl->ints(k, es, ee);
   const int k0 = k;
   const int kst = k;
-  const V3<T> p0 = p;
-  // precise mode: the position follows from the step index (no drift from
-  // repeated p += inc); reference mode: the incrementally updated p.
+  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+  // Sample coordinates are carried half-texel shifted (h = p - 0.5).
+  // Precise mode: h follows from the step index, (p0 - 0.5) + t inc with one
+  // rounding per axis (no drift from repeated p += inc); reference mode: the
+  // incrementally updated p, minus 0.5.
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return p;
+    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{p0.x + t * inc.x, p0.y + t * inc.y, p0.z + t * inc.z};
+    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
   };
   int a = 0, b = 0;
   while (k > 0) {
@@ -899,12 +906,12 @@ long long orc_count_samples(const orc_volume* vol, const orc_view* view, const o
 
 double orc_sample_f64(const orc_volume* vol, double px, double py, double pz) {
   double out = 0.0;
-  with_grid(vol, [&](auto g) { out = trilinear<double>(g, px, py, pz); return 0; });
+  with_grid(vol, [&](auto g) { out = trilinear<double>(g, px - 0.5, py - 0.5, pz - 0.5); return 0; });
   return out;
 }
 float orc_sample_f32(const orc_volume* vol, float px, float py, float pz) {
   float out = 0.0f;
-  with_grid(vol, [&](auto g) { out = trilinear<float>(g, px, py, pz); return 0; });
+  with_grid(vol, [&](auto g) { out = trilinear<float>(g, px - 0.5f, py - 0.5f, pz - 0.5f); return 0; });
   return out;
 }
 

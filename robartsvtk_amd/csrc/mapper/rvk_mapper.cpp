@@ -3,7 +3,7 @@
 // Every device operation goes through the librvkhip C-ABI.  Paths below are
 // relative to the reference root; V/ = Visualization/vtkCudaVisualization,
 // C/ = Common/vtkCudaCommon.
-#include "../../../include/rvk_mapper.h"
+#include "rvk_mapper.h"
 
 #include <algorithm>
 #include <cmath>

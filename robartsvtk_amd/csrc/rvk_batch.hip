@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void jac_grad_kernel(JacGradArgs a) {
 // every detector, intersects the covered rays with the support, and re-marches
 // only those samples; one thread owns one voxel, so there are no atomics.
 struct RayRec {
-  float p0x, p0y, p0z, ix, iy, iz;
+  float p0x, p0y, p0z, ix, iy, iz;  // shifted start p0 - 0.5 and step
   int n;
   float cs, ci;   // alpha = cs * value + ci
   float coef;     // d loss / d L_p * ds
@@ -397,9 +397,9 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
   const float ds = sqrtf(r.inc.x * r.inc.x * g.sp[0] * g.sp[0] + r.inc.y * r.inc.y * g.sp[1] * g.sp[1] +
                          r.inc.z * r.inc.z * g.sp[2] * g.sp[2]);
   RayRec rr;
-  rr.p0x = r.start.x + j * r.inc.x;
-  rr.p0y = r.start.y + j * r.inc.y;
-  rr.p0z = r.start.z + j * r.inc.z;
+  rr.p0x = (r.start.x + j * r.inc.x) - 0.5f;
+  rr.p0y = (r.start.y + j * r.inc.y) - 0.5f;
+  rr.p0z = (r.start.z + j * r.inc.z) - 0.5f;
   rr.ix = r.inc.x; rr.iy = r.inc.y; rr.iz = r.inc.z;
   rr.n = step_count(r.num_steps - j);
   rr.ci = a.b.m.intercept * -ds;
@@ -432,13 +432,16 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
         for (int dx = -1; dx <= 1; ++dx) {
           const int xi = min(max(i + dx, 0), g.n[0] - 1), yi = min(max(jj + dy, 0), g.n[1] - 1);
           const int zi = min(max(k + dz, 0), g.n[2] - 1);
-          mx = fmaxf(mx, vol[brick_x(xi) + brick_y(yi, g.bys) + brick_z(zi, g.bzs)]);
+          mx = fmaxf(mx, load_voxel(vol, g, xi, yi, zi));
         }
     live = mx > 0.0f;
   }
   float acc = 0.0f;
   if (live) {
+    // the support in sample coordinates p: (i-0.5, i+1.5)^3; lo is its low
+    // corner, hl the same corner in shifted coordinates h = p - 0.5
     const float lo[3] = {(float)i - 0.5f, (float)jj - 0.5f, (float)k - 0.5f};
+    const float hl[3] = {(float)i - 1.0f, (float)jj - 1.0f, (float)k - 1.0f};
     for (int v = 0; v < a.b.n_views; ++v) {
       const BatchView& bv = a.b.views[v];
       const int W = bv.v.W, H = bv.v.H;
@@ -468,26 +471,26 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
         for (int px = x0; px <= x1; ++px) {
           const RayRec rr = a.rays[bv.pix_off + (long long)py * W + px];
           if (rr.n <= 0 || rr.coef == 0.0f) continue;
-          // sample indices m with p0 + m inc inside the open support box
+          // sample indices m with h0 + m inc inside the open support box
           float tlo = 0.0f, thi = (float)(rr.n - 1);
           const float p0[3] = {rr.p0x, rr.p0y, rr.p0z}, in[3] = {rr.ix, rr.iy, rr.iz};
 #pragma unroll
           for (int ax = 0; ax < 3; ++ax) {
             if (in[ax] != 0.0f) {
-              const float ta = (lo[ax] - p0[ax]) / in[ax], tb = (lo[ax] + 2.0f - p0[ax]) / in[ax];
+              const float ta = (hl[ax] - p0[ax]) / in[ax], tb = (hl[ax] + 2.0f - p0[ax]) / in[ax];
               tlo = fmaxf(tlo, fminf(ta, tb));
               thi = fminf(thi, fmaxf(ta, tb));
-            } else if (p0[ax] <= lo[ax] || p0[ax] >= lo[ax] + 2.0f) {
+            } else if (p0[ax] <= hl[ax] || p0[ax] >= hl[ax] + 2.0f) {
               thi = -1.0f;
             }
           }
           const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(rr.n - 1, (int)ceilf(thi) + 1);
           for (int m = m0; m <= m1; ++m) {
             const float t = (float)m;
-            const float qx = rr.p0x + t * rr.ix, qy = rr.p0y + t * rr.iy, qz = rr.p0z + t * rr.iz;
-            const float w = hat(qx - 0.5f - (float)i) * hat(qy - 0.5f - (float)jj) * hat(qz - 0.5f - (float)k);
+            const float qx = fmaf(t, rr.ix, rr.p0x), qy = fmaf(t, rr.iy, rr.p0y), qz = fmaf(t, rr.iz, rr.p0z);
+            const float w = hat(qx - (float)i) * hat(qy - (float)jj) * hat(qz - (float)k);
             if (w <= 0.0f) continue;
-            const float alpha = rr.cs * trilinear(vol, g, qx, qy, qz) + rr.ci;
+            const float alpha = fmaf(rr.cs, trilinear(vol, g, qx, qy, qz), rr.ci);
             if (alpha < 0.0f) acc += rr.coef * w;
           }
         }

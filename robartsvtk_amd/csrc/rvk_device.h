@@ -6,11 +6,14 @@
 //   volume texture ..... unnormalised / linear / clamp (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:305-316)
 //   1D TF texture ...... normalised / linear / clamp  (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:248-267)
 //   2D TF texture ...... normalised / point / clamp   (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:21-25)
-// The texture unit is replaced by explicit gathers from a native-type,
-// 4x4x4-bricked volume with exact FP32 weights.
+// The texture unit is replaced by one vector load per sample from a
+// native-type cell volume (every 2x2x2 trilinear footprint stored whole) with
+// exact FP32 weights.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 namespace rvk {
 
@@ -18,7 +21,7 @@ namespace rvk {
 // Parameters every caster receives by value (no __constant__ module state).
 struct VolGeom {
   int n[3];        // dims
-  int bys, bzs;    // brick-row / brick-slab strides in elements (64*nbx, 64*nbx*nby)
+  uint32_t cys, czs;  // cell-brick row / slab strides in cells
   float srecip[3]; // 0.5/spacing  (V/vtkCudaVolumeInformationHandler.cxx:91-93)
   float sp[3];     // spacing
   float bounds[6]; // [0, dim-1] per axis
@@ -70,87 +73,165 @@ __device__ __forceinline__ uint32_t to_u8(float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Bricked addressing: voxel (i,j,k) lives at brick (i>>2, j>>2, k>>2), offset
-// (i&3) + 4*(j&3) + 16*(k&3) inside a 64-voxel brick.  The address separates
-// per axis, so the 8 corners of a trilinear cell cost 6 axis terms + 8 adds.
-__device__ __forceinline__ int brick_x(int i) { return ((i >> 2) << 6) | (i & 3); }
-__device__ __forceinline__ int brick_y(int j, int bys) { return (j >> 2) * bys + ((j & 3) << 2); }
-__device__ __forceinline__ int brick_z(int k, int bzs) { return (k >> 2) * bzs + ((k & 3) << 4); }
+// Cell layout.  A trilinear reconstruction reads the 2x2x2 voxels of one cell,
+// so every cell is stored whole: cell (cx,cy,cz) holds the 8 voxels
+// v[clamp(cx-1+dx)][clamp(cy-1+dy)][clamp(cz-1+dz)], dx,dy,dz in {0,1}, in
+// the order dx + 2 dy + 4 dz, as one 8/16/32-byte vector (u8/16-bit/32-bit
+// voxels).  Cell indices run over [0, n+1] per axis (the sample coordinate is
+// clamped to [-1, n] first), so edge clamping is baked into the cells and a
+// sample costs one vector load and no index clamps.  The volume grows 8x; the
+// cells are grouped in bricks of (2^RVK_CELL_BRICK_LOG2)^3 with separable
+// addressing; on the c2 DRR (256^3 i16, 1024^2) plain x-fastest cell order
+// (edge 1) measured 759 G samples/s against 667 (edge 2) and 681 (edge 4).
+#ifndef RVK_CELL_BRICK_LOG2
+#define RVK_CELL_BRICK_LOG2 0
+#endif
+constexpr int kCellBrickLog2 = RVK_CELL_BRICK_LOG2;
+constexpr int kCellBrickEdge = 1 << kCellBrickLog2;
+constexpr int kCellBrickCells = kCellBrickEdge * kCellBrickEdge * kCellBrickEdge;
 
-template <typename VT> __device__ __forceinline__ float ldv(const VT* __restrict__ d, int off) {
-  return (float)d[off];
+__device__ __forceinline__ uint32_t cell_x(int c) {
+  return (uint32_t)(((c >> kCellBrickLog2) * kCellBrickCells) | (c & (kCellBrickEdge - 1)));
+}
+// The row term stays below the slab stride czs <= 2^24 (checked when a volume
+// is created), so it takes the full-rate 24-bit multiply.
+__device__ __forceinline__ uint32_t cell_y(int c, uint32_t cys) {
+  return __umul24((uint32_t)(c >> kCellBrickLog2), cys) + (uint32_t)((c & (kCellBrickEdge - 1)) << kCellBrickLog2);
+}
+__device__ __forceinline__ uint32_t cell_z(int c, uint32_t czs) {
+  return (uint32_t)(c >> kCellBrickLog2) * czs + (uint32_t)((c & (kCellBrickEdge - 1)) << (2 * kCellBrickLog2));
+}
+__device__ __forceinline__ uint32_t cell_index(const VolGeom& g, int cx, int cy, int cz) {
+  return cell_x(cx) + cell_y(cy, g.cys) + cell_z(cz, g.czs);
 }
 
-// tex3D(unnormalised, linear, clamp) at p: trilinear at p-0.5 with indices
-// clamped to [0, n-1].  Clamping the coordinate to [-1, n] first leaves every
-// clamp-addressed result unchanged and keeps the integer conversion in range.
+// The 8 corner values of a cell as floats, in the order dx + 2 dy + 4 dz.
+template <typename VT> struct CellLoad;
+template <> struct CellLoad<uint8_t> {
+  static __device__ __forceinline__ void get(const uint8_t* __restrict__ d, uint32_t c, float v[8]) {
+    const uint2 w = *reinterpret_cast<const uint2*>(d + (size_t)c * 8);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = (float)((w.x >> (8 * i)) & 0xffu);
+      v[4 + i] = (float)((w.y >> (8 * i)) & 0xffu);
+    }
+  }
+};
+template <> struct CellLoad<int8_t> {
+  static __device__ __forceinline__ void get(const int8_t* __restrict__ d, uint32_t c, float v[8]) {
+    const int2 w = *reinterpret_cast<const int2*>(d + (size_t)c * 8);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = (float)((int)((uint32_t)w.x << (24 - 8 * i)) >> 24);
+      v[4 + i] = (float)((int)((uint32_t)w.y << (24 - 8 * i)) >> 24);
+    }
+  }
+};
+template <> struct CellLoad<uint16_t> {
+  static __device__ __forceinline__ void get(const uint16_t* __restrict__ d, uint32_t c, float v[8]) {
+    const uint4 w = *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
+    const uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[2 * i] = (float)(q[i] & 0xffffu);
+      v[2 * i + 1] = (float)(q[i] >> 16);
+    }
+  }
+};
+template <> struct CellLoad<int16_t> {
+  static __device__ __forceinline__ void get(const int16_t* __restrict__ d, uint32_t c, float v[8]) {
+    const int4 w = *reinterpret_cast<const int4*>(d + (size_t)c * 8);
+    const int q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[2 * i] = (float)((int)((uint32_t)q[i] << 16) >> 16);
+      v[2 * i + 1] = (float)(q[i] >> 16);
+    }
+  }
+};
+template <typename W> struct CellLoad32 {
+  static __device__ __forceinline__ void get(const W* __restrict__ d, uint32_t c, float v[8]) {
+    const W* p = d + (size_t)c * 8;
+    const auto a = *reinterpret_cast<const typename std::conditional<std::is_same<W, float>::value, float4,
+                                     typename std::conditional<std::is_same<W, int32_t>::value, int4, uint4>::type>::type*>(p);
+    const auto b = *(reinterpret_cast<const decltype(a)*>(p) + 1);
+    v[0] = (float)a.x; v[1] = (float)a.y; v[2] = (float)a.z; v[3] = (float)a.w;
+    v[4] = (float)b.x; v[5] = (float)b.y; v[6] = (float)b.z; v[7] = (float)b.w;
+  }
+};
+template <> struct CellLoad<float> : CellLoad32<float> {};
+template <> struct CellLoad<int32_t> : CellLoad32<int32_t> {};
+template <> struct CellLoad<uint32_t> : CellLoad32<uint32_t> {};
+
+// Voxel (i,j,k) (in range) = corner 000 of cell (i+1, j+1, k+1).
 template <typename VT>
-__device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGeom& g, float px, float py, float pz) {
-  float x = px - 0.5f, y = py - 0.5f, z = pz - 0.5f;
-  x = fminf(fmaxf(x, -1.0f), (float)g.n[0]);
-  y = fminf(fmaxf(y, -1.0f), (float)g.n[1]);
-  z = fminf(fmaxf(z, -1.0f), (float)g.n[2]);
+__device__ __forceinline__ float load_voxel(const VT* __restrict__ d, const VolGeom& g, int i, int j, int k) {
+  return (float)d[(size_t)cell_index(g, i + 1, j + 1, k + 1) * 8];
+}
+
+// Cell and weights of tex3D(unnormalised, linear, clamp) at p, given the
+// half-texel-shifted coordinate h = p - 0.5: trilinear at h with indices
+// clamped to [0, n-1].  Clamping h to [-1, n] first leaves every
+// clamp-addressed result unchanged and keeps the integer conversion in range
+// (sample coordinates are finite: rays with non-finite steps take no samples).
+struct CellPos {
+  uint32_t cell;
+  float ax, ay, az;
+  bool xs, ys, zs;  // the two corners along the axis are distinct voxels
+};
+__device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float hx, float hy, float hz) {
+  const float x = __builtin_amdgcn_fmed3f(hx, -1.0f, (float)g.n[0]);
+  const float y = __builtin_amdgcn_fmed3f(hy, -1.0f, (float)g.n[1]);
+  const float z = __builtin_amdgcn_fmed3f(hz, -1.0f, (float)g.n[2]);
   const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
-  const float ax = x - fx, ay = y - fy, az = z - fz;
+  CellPos c;
+  c.ax = x - fx;
+  c.ay = y - fy;
+  c.az = z - fz;
   const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-  const int x0 = min(max(ix, 0), g.n[0] - 1), x1 = min(max(ix + 1, 0), g.n[0] - 1);
-  const int y0 = min(max(iy, 0), g.n[1] - 1), y1 = min(max(iy + 1, 0), g.n[1] - 1);
-  const int z0 = min(max(iz, 0), g.n[2] - 1), z1 = min(max(iz + 1, 0), g.n[2] - 1);
-  const int ox0 = brick_x(x0), ox1 = brick_x(x1);
-  const int oy0 = brick_y(y0, g.bys), oy1 = brick_y(y1, g.bys);
-  const int oz0 = brick_z(z0, g.bzs), oz1 = brick_z(z1, g.bzs);
-  const float v000 = ldv(d, ox0 + oy0 + oz0), v100 = ldv(d, ox1 + oy0 + oz0);
-  const float v010 = ldv(d, ox0 + oy1 + oz0), v110 = ldv(d, ox1 + oy1 + oz0);
-  const float v001 = ldv(d, ox0 + oy0 + oz1), v101 = ldv(d, ox1 + oy0 + oz1);
-  const float v011 = ldv(d, ox0 + oy1 + oz1), v111 = ldv(d, ox1 + oy1 + oz1);
-  const float c00 = v000 + ax * (v100 - v000);
-  const float c10 = v010 + ax * (v110 - v010);
-  const float c01 = v001 + ax * (v101 - v001);
-  const float c11 = v011 + ax * (v111 - v011);
-  const float c0 = c00 + ay * (c10 - c00);
-  const float c1 = c01 + ay * (c11 - c01);
-  return c0 + az * (c1 - c0);
+  c.cell = cell_index(g, ix + 1, iy + 1, iz + 1);
+  c.xs = ix >= 0 && ix < g.n[0] - 1;
+  c.ys = iy >= 0 && iy < g.n[1] - 1;
+  c.zs = iz >= 0 && iz < g.n[2] - 1;
+  return c;
+}
+
+// a + t (b - a) as one rounding of the product-sum
+__device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+// Trilinear value at the shifted coordinate h = p - 0.5.
+template <typename VT>
+__device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGeom& g, float hx, float hy, float hz) {
+  const CellPos c = cell_pos(g, hx, hy, hz);
+  float v[8];
+  CellLoad<VT>::get(d, c.cell, v);
+  const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
+  const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
+  const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
+  return lerp(c0, c1, c.az);
 }
 
 // Trilinear value and its gradient with respect to p (voxel index units) on
-// the same clamped cell.  The value is computed with exactly the operations
-// of trilinear(), so branch decisions made on it match the plain caster.
+// the same clamped cell, at the shifted coordinate h = p - 0.5.  The value is
+// computed with exactly the operations of trilinear(), so branch decisions
+// made on it match the plain caster.
 // d/dx = lerp_z(lerp_y(v100-v000, v110-v010), lerp_y(v101-v001, v111-v011)),
 // d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0; axes whose two corners
 // clamp to one voxel get a zero derivative.
 template <typename VT>
-__device__ __forceinline__ float trilinear_vg(const VT* __restrict__ d, const VolGeom& g, float px, float py, float pz,
+__device__ __forceinline__ float trilinear_vg(const VT* __restrict__ d, const VolGeom& g, float hx, float hy, float hz,
                                               V3<float>& gr) {
-  float x = px - 0.5f, y = py - 0.5f, z = pz - 0.5f;
-  x = fminf(fmaxf(x, -1.0f), (float)g.n[0]);
-  y = fminf(fmaxf(y, -1.0f), (float)g.n[1]);
-  z = fminf(fmaxf(z, -1.0f), (float)g.n[2]);
-  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
-  const float ax = x - fx, ay = y - fy, az = z - fz;
-  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-  const int x0 = min(max(ix, 0), g.n[0] - 1), x1 = min(max(ix + 1, 0), g.n[0] - 1);
-  const int y0 = min(max(iy, 0), g.n[1] - 1), y1 = min(max(iy + 1, 0), g.n[1] - 1);
-  const int z0 = min(max(iz, 0), g.n[2] - 1), z1 = min(max(iz + 1, 0), g.n[2] - 1);
-  const int ox0 = brick_x(x0), ox1 = brick_x(x1);
-  const int oy0 = brick_y(y0, g.bys), oy1 = brick_y(y1, g.bys);
-  const int oz0 = brick_z(z0, g.bzs), oz1 = brick_z(z1, g.bzs);
-  const float v000 = ldv(d, ox0 + oy0 + oz0), v100 = ldv(d, ox1 + oy0 + oz0);
-  const float v010 = ldv(d, ox0 + oy1 + oz0), v110 = ldv(d, ox1 + oy1 + oz0);
-  const float v001 = ldv(d, ox0 + oy0 + oz1), v101 = ldv(d, ox1 + oy0 + oz1);
-  const float v011 = ldv(d, ox0 + oy1 + oz1), v111 = ldv(d, ox1 + oy1 + oz1);
-  const float c00 = v000 + ax * (v100 - v000);
-  const float c10 = v010 + ax * (v110 - v010);
-  const float c01 = v001 + ax * (v101 - v001);
-  const float c11 = v011 + ax * (v111 - v011);
-  const float c0 = c00 + ay * (c10 - c00);
-  const float c1 = c01 + ay * (c11 - c01);
-  const float dx00 = v100 - v000, dx10 = v110 - v010, dx01 = v101 - v001, dx11 = v111 - v011;
-  const float dx0 = dx00 + ay * (dx10 - dx00), dx1 = dx01 + ay * (dx11 - dx01);
-  const float dy0 = c10 - c00, dy1 = c11 - c01;
-  gr.x = (x1 != x0) ? dx0 + az * (dx1 - dx0) : 0.0f;
-  gr.y = (y1 != y0) ? dy0 + az * (dy1 - dy0) : 0.0f;
-  gr.z = (z1 != z0) ? c1 - c0 : 0.0f;
-  return c0 + az * (c1 - c0);
+  const CellPos c = cell_pos(g, hx, hy, hz);
+  float v[8];
+  CellLoad<VT>::get(d, c.cell, v);
+  const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
+  const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
+  const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
+  const float dx0 = lerp(v[1] - v[0], v[3] - v[2], c.ay), dx1 = lerp(v[5] - v[4], v[7] - v[6], c.ay);
+  gr.x = c.xs ? lerp(dx0, dx1, c.az) : 0.0f;
+  gr.y = c.ys ? lerp(c10 - c00, c11 - c01, c.az) : 0.0f;
+  gr.z = c.zs ? c1 - c0 : 0.0f;
+  return lerp(c0, c1, c.az);
 }
 
 // tex1D(normalised, linear, clamp) on an n-entry table.
@@ -161,8 +242,7 @@ __device__ __forceinline__ float tf1d_lookup(const float* t, int n, float u) {
   const float a = x - fx;
   const int i = (int)fx;
   const int i0 = min(max(i, 0), n - 1), i1 = min(max(i + 1, 0), n - 1);
-  const float v0 = t[i0], v1 = t[i1];
-  return v0 + a * (v1 - v0);
+  return lerp(t[i0], t[i1], a);
 }
 
 // tex2D(normalised, point, clamp) cell index g*n + i.

@@ -9,7 +9,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "../../include/rvk_geometry.h"
+#include "rvk_geometry.h"
 
 namespace {
 

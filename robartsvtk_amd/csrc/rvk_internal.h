@@ -4,7 +4,7 @@
 
 #include <string>
 
-#include "../../include/rvk.h"
+#include "rvk.h"
 #include "rvk_device.h"
 
 struct rvk_ctx {
@@ -22,10 +22,10 @@ struct rvk_volume {
   double spacing[3] = {1, 1, 1};
   int dtype = RVK_U8;
   int n_frames = 0;
-  int nb[3] = {0, 0, 0};         // brick grid
-  size_t padded_elems = 0;       // 64 * nb[0]*nb[1]*nb[2]
-  void* frames[RVK_MAX_FRAMES] = {};
-  uint8_t* gradmag[RVK_MAX_FRAMES] = {};
+  int ncb[3] = {0, 0, 0};        // cell-brick grid: ceil((dims+2)/brick edge)
+  size_t n_cells = 0;            // cells incl. brick padding (each 8 voxels)
+  void* frames[RVK_MAX_FRAMES] = {};    // cell volumes, n_cells * 8 voxels
+  uint8_t* gradmag[RVK_MAX_FRAMES] = {};  // u8 cell volumes
   float gradmag_scale[RVK_MAX_FRAMES] = {};
 };
 

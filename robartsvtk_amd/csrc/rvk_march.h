@@ -55,16 +55,21 @@ __device__ __forceinline__ void step(V3<float>& p, const V3<float>& inc) {
   p.x += inc.x; p.y += inc.y; p.z += inc.z;
 }
 
-// Sample position.  Reference mode keeps the incrementally updated p
-// (p += inc per step, as the CUDA kernels do); PRECISE derives it from the
-// step index, p0 + (k_start - k) inc, which removes the drift that repeated
-// FP32 additions build up over long rays.
+// Sample coordinate, shifted by the half texel (h = p - 0.5, what trilinear()
+// takes).  Reference mode keeps the incrementally updated p (p += inc per
+// step, as the CUDA kernels do) and subtracts 0.5; PRECISE derives it from
+// the step index, h = (p0 - 0.5) + (k_start - k) inc with one rounding per
+// axis, which removes the drift repeated FP32 additions build up.
 template <bool PRECISE>
-__device__ __forceinline__ V3<float> cur_pos(const V3<float>& p, const V3<float>& p0, const V3<float>& inc, int kst,
-                                             int k) {
-  if (!PRECISE) return p;
+__device__ __forceinline__ V3<float> sample_pos(const V3<float>& p, const V3<float>& p0h, const V3<float>& inc, int kst,
+                                                int k) {
+  if (!PRECISE) return V3<float>{p.x - 0.5f, p.y - 0.5f, p.z - 0.5f};
   const float t = (float)(kst - k);
-  return V3<float>{p0.x + t * inc.x, p0.y + t * inc.y, p0.z + t * inc.z};
+  return V3<float>{fmaf(t, inc.x, p0h.x), fmaf(t, inc.y, p0h.y), fmaf(t, inc.z, p0h.z)};
+}
+
+__device__ __forceinline__ V3<float> half_shift(const V3<float>& p) {
+  return V3<float>{p.x - 0.5f, p.y - 0.5f, p.z - 0.5f};
 }
 
 // ===========================================================================
@@ -79,10 +84,11 @@ struct NoVisit {
 };
 
 // Line integral along one ray.  The accumulator starts at ln(256*offset) in
-// the reference RGBA8 mode and at 0 in the float modes.  `visit(q, t, alpha,
-// grad)` is called for every accumulated sample, at position q = p0 + t*inc in
-// PRECISE mode; with VG the trilinear gradient at q comes from the same
-// gathers as the value (the adjoint marches through this very control flow).
+// the reference RGBA8 mode and at 0 in the float modes.  `visit(h, t, alpha,
+// grad)` is called for every accumulated sample, with h its shifted sample
+// coordinate (h = p0 - 0.5 + t*inc in PRECISE mode); with VG the trilinear
+// gradient at h comes from the same gathers as the value (the adjoint marches
+// through this very control flow).
 template <typename VT, bool SKIP, bool PRECISE, bool VG = false, typename F = NoVisit>
 __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const VolGeom& g, const DrrMarchArgs& a,
                                            const Ray& r, int x, int y, F&& visit = F()) {
@@ -100,7 +106,7 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
   const int es = k - (int)ceilf(r.exclude_start);
   const int ee = k - (int)floorf(r.exclude_end);
   const int kst = k;
-  const V3<float> p0 = p;
+  const V3<float> p0h = half_shift(p);
   float comp = 0.0f;  // Kahan compensation of the line integral (PRECISE)
   auto add = [&](float v) {
     if (!PRECISE) { acc += v; return; }
@@ -112,9 +118,9 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
   int sa = 0, sb = 0;
   while (k > 0) {
     const bool in_key = (es >= k && ee < k);
-    const V3<float> q = cur_pos<PRECISE>(p, p0, inc, kst, k);
+    const V3<float> q = sample_pos<PRECISE>(p, p0h, inc, kst, k);
     V3<float> gr{0.0f, 0.0f, 0.0f};
-    const float alpha = cs * (VG ? trilinear_vg(vol, g, q.x, q.y, q.z, gr) : trilinear(vol, g, q.x, q.y, q.z)) + ci;
+    const float alpha = fmaf(cs, VG ? trilinear_vg(vol, g, q.x, q.y, q.z, gr) : trilinear(vol, g, q.x, q.y, q.z), ci);
     const float t = (float)(kst - k);
     if (SKIP) {
       if (alpha < 0.0f && !in_key) {

@@ -6,7 +6,8 @@
 //   V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:584-651 (CUDA_castBuffer: host
 //     copy + widening to float) and the *_loadImageInfo 3D-array uploads
 //     (e.g. V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:409-443): volumes stay
-//     in their native type and are relaid out into 4x4x4 bricks on the device;
+//     in their native type and are relaid out on the device into cells that
+//     hold each trilinear 2x2x2 footprint whole (8x the voxel bytes);
 //   the TF loadTextures functions (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:324-372,
 //     V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:589-651).
 #include <cmath>
@@ -57,8 +58,8 @@ VolGeom vol_geom(const rvk_volume* v) {
     g.bounds[2 * a] = 0.0f;
     g.bounds[2 * a + 1] = (float)v->dims[a] - 1.0f;
   }
-  g.bys = 64 * v->nb[0];
-  g.bzs = 64 * v->nb[0] * v->nb[1];
+  g.cys = (uint32_t)(kCellBrickCells * v->ncb[0]);
+  g.czs = (uint32_t)(kCellBrickCells * v->ncb[0] * v->ncb[1]);
   return g;
 }
 
@@ -115,47 +116,63 @@ int ensure_scratch(rvk_ctx* ctx, size_t bytes) {
 }
 
 // ---------------------------------------------------------------------------
-// Linear (x fastest) -> 4x4x4 bricks.  One thread per bricked element, so the
-// stores are dense; padding voxels replicate the nearest edge voxel.
+// Linear (x fastest) -> cells.  One thread per cell: it gathers the cell's 8
+// clamped voxels and writes them as one vector.  Cells past [0, n+1] (brick
+// padding) are never sampled; they repeat the nearest edge.
+template <typename VT> struct CellVec;
+template <> struct CellVec<uint8_t> { using T = uint2; };
+template <> struct CellVec<int8_t> { using T = uint2; };
+template <> struct CellVec<uint16_t> { using T = uint4; };
+template <> struct CellVec<int16_t> { using T = uint4; };
+
 template <typename VT>
-__global__ __launch_bounds__(256) void brick_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst,
-                                                             int nx, int ny, int nz, int nbx, int nby,
-                                                             long long total) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= total) return;
-  const long long b = e >> 6;
-  const int o = (int)(e & 63);
-  const int bx = (int)(b % nbx);
-  const int by = (int)((b / nbx) % nby);
-  const int bz = (int)(b / ((long long)nbx * nby));
-  int i = bx * 4 + (o & 3), j = by * 4 + ((o >> 2) & 3), k = bz * 4 + (o >> 4);
-  i = min(i, nx - 1);
-  j = min(j, ny - 1);
-  k = min(k, nz - 1);
-  dst[e] = src[(long long)i + (long long)j * nx + (long long)k * nx * ny];
+__global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst, int nx,
+                                                            int ny, int nz, int ncbx, int ncby, long long n_cells) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  const long long b = c / kCellBrickCells;
+  const int o = (int)(c % kCellBrickCells);
+  const int bx = (int)(b % ncbx);
+  const int by = (int)((b / ncbx) % ncby);
+  const int bz = (int)(b / ((long long)ncbx * ncby));
+  const int m = kCellBrickEdge - 1;
+  const int cx = bx * kCellBrickEdge + (o & m);
+  const int cy = by * kCellBrickEdge + ((o >> kCellBrickLog2) & m);
+  const int cz = bz * kCellBrickEdge + (o >> (2 * kCellBrickLog2));
+  alignas(16) VT v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int i = min(max(cx - 1 + (q & 1), 0), nx - 1);
+    const int j = min(max(cy - 1 + ((q >> 1) & 1), 0), ny - 1);
+    const int k = min(max(cz - 1 + (q >> 2), 0), nz - 1);
+    v[q] = src[(long long)i + (long long)j * nx + (long long)k * nx * ny];
+  }
+  VT* out = dst + c * 8;
+  if constexpr (sizeof(VT) == 4) {
+    reinterpret_cast<uint4*>(out)[0] = *reinterpret_cast<const uint4*>(&v[0]);
+    reinterpret_cast<uint4*>(out)[1] = *reinterpret_cast<const uint4*>(&v[4]);
+  } else {
+    *reinterpret_cast<typename CellVec<VT>::T*>(out) = *reinterpret_cast<const typename CellVec<VT>::T*>(&v[0]);
+  }
 }
 
 // Gradient magnitude by central differences at voxel centres, the value the
 // reference's on-the-fly gradient takes there (tex3D at p +- 1 with clamp,
 // times 0.5/spacing; V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:263-271),
-// quantised to u8 as round(|g| / scale).
+// quantised to u8 as round(|g| / scale), written x-fastest.
 template <typename VT>
 __global__ __launch_bounds__(256) void gradmag_kernel(const VT* __restrict__ vol, uint8_t* __restrict__ out,
                                                       VolGeom g, float inv_scale, long long total) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total) return;
-  const long long b = e >> 6;
-  const int o = (int)(e & 63);
-  const int nbx = g.bys >> 6, nby = (g.bzs / g.bys);
-  const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
-  const int i = min(bx * 4 + (o & 3), g.n[0] - 1);
-  const int j = min(by * 4 + ((o >> 2) & 3), g.n[1] - 1);
-  const int k = min(bz * 4 + (o >> 4), g.n[2] - 1);
+  const int i = (int)(e % g.n[0]);
+  const int j = (int)((e / g.n[0]) % g.n[1]);
+  const int k = (int)(e / ((long long)g.n[0] * g.n[1]));
   auto at = [&](int x, int y, int z) {
     x = min(max(x, 0), g.n[0] - 1);
     y = min(max(y, 0), g.n[1] - 1);
     z = min(max(z, 0), g.n[2] - 1);
-    return (float)vol[brick_x(x) + brick_y(y, g.bys) + brick_z(z, g.bzs)];
+    return load_voxel(vol, g, x, y, z);
   };
   const float gx = (at(i + 1, j, k) - at(i - 1, j, k)) * g.srecip[0];
   const float gy = (at(i, j + 1, k) - at(i, j - 1, k)) * g.srecip[1];
@@ -166,22 +183,22 @@ __global__ __launch_bounds__(256) void gradmag_kernel(const VT* __restrict__ vol
 
 template <typename VT>
 static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
-  const long long total = (long long)v->padded_elems;
+  const long long total = (long long)v->n_cells;
   const int blocks = (int)((total + 255) / 256);
-  hipLaunchKernelGGL(brick_relayout_kernel<VT>, dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc,
-                     (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->nb[0], v->nb[1], total);
+  hipLaunchKernelGGL(cell_relayout_kernel<VT>, dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc,
+                     (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
   RVK_HIP(hipGetLastError());
   return RVK_OK;
 }
 
 template <typename VT>
-static int launch_gradmag(rvk_volume* v, int frame, float inv_scale) {
-  const long long total = (long long)v->padded_elems;
+static int launch_gradmag(rvk_volume* v, int frame, float inv_scale, uint8_t* linear) {
+  const long long total = (long long)v->dims[0] * v->dims[1] * v->dims[2];
   const int blocks = (int)((total + 255) / 256);
   hipLaunchKernelGGL(gradmag_kernel<VT>, dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)v->frames[frame],
-                     v->gradmag[frame], vol_geom(v), inv_scale, total);
+                     linear, vol_geom(v), inv_scale, total);
   RVK_HIP(hipGetLastError());
-  return RVK_OK;
+  return launch_relayout<uint8_t>(v, linear, v->gradmag[frame]);
 }
 
 template <typename F> static int dispatch_dtype(int dtype, F&& f) {
@@ -346,12 +363,17 @@ int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[
   for (int a = 0; a < 3; ++a) {
     v->dims[a] = dims[a];
     v->spacing[a] = spacing[a];
-    v->nb[a] = (dims[a] + 3) / 4;
+    v->ncb[a] = (dims[a] + 2 + kCellBrickEdge - 1) / kCellBrickEdge;
   }
-  v->padded_elems = (size_t)64 * v->nb[0] * v->nb[1] * v->nb[2];
-  if (v->padded_elems > (size_t)0x7fffffff) {
+  v->n_cells = (size_t)kCellBrickCells * v->ncb[0] * v->ncb[1] * v->ncb[2];
+  const size_t slab = (size_t)kCellBrickCells * v->ncb[0] * v->ncb[1];
+  if (slab > ((size_t)1 << 24)) {
     delete v;
-    return fail(RVK_ERR_INVALID_ARGUMENT, "volume too large for 32-bit element offsets");
+    return fail(RVK_ERR_INVALID_ARGUMENT, "volume x/y extent too large: %zu cells per z slab exceed 2^24", slab);
+  }
+  if (v->n_cells > (size_t)0xffffffffu) {
+    delete v;
+    return fail(RVK_ERR_INVALID_ARGUMENT, "volume too large: %zu cells exceed 32-bit cell indices", v->n_cells);
   }
   *out = v;
   return RVK_OK;
@@ -364,7 +386,7 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
   RVK_HIP(hipSetDevice(v->ctx->device));
   const size_t es = dtype_size(v->dtype);
   const size_t n = (size_t)v->dims[0] * v->dims[1] * v->dims[2];
-  if (!v->frames[frame]) RVK_HIP(hipMalloc(&v->frames[frame], v->padded_elems * es));
+  if (!v->frames[frame]) RVK_HIP(hipMalloc(&v->frames[frame], v->n_cells * 8 * es));
   const void* dsrc = src;
   void* staging = nullptr;
   if (!src_on_device) {
@@ -391,13 +413,19 @@ int rvk_volume_compute_gradmag(rvk_volume* v, int frame, float max_gradient) {
   if (!(max_gradient > 0.0f) || !std::isfinite(max_gradient))
     return fail(RVK_ERR_INVALID_ARGUMENT, "max_gradient must be positive and finite");
   RVK_HIP(hipSetDevice(v->ctx->device));
-  if (!v->gradmag[frame]) RVK_HIP(hipMalloc((void**)&v->gradmag[frame], v->padded_elems));
+  if (!v->gradmag[frame]) RVK_HIP(hipMalloc((void**)&v->gradmag[frame], v->n_cells * 8));
   v->gradmag_scale[frame] = max_gradient / 255.0f;
   const float inv = 255.0f / max_gradient;
-  return dispatch_dtype(v->dtype, [&](auto tag) {
+  uint8_t* linear = nullptr;
+  RVK_HIP(hipMalloc((void**)&linear, (size_t)v->dims[0] * v->dims[1] * v->dims[2]));
+  rc = dispatch_dtype(v->dtype, [&](auto tag) {
     using VT = std::remove_pointer_t<decltype(tag)>;
-    return launch_gradmag<VT>(v, frame, inv);
+    return launch_gradmag<VT>(v, frame, inv, linear);
   });
+  const hipError_t e = hipStreamSynchronize(v->ctx->stream);
+  hipFree(linear);
+  if (rc == RVK_OK && e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return rc;
 }
 
 int rvk_volume_destroy(rvk_volume* v) {

@@ -5,7 +5,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/rvk_synth.h"
+#include "rvk_synth.h"
 
 namespace {
 

@@ -315,3 +315,28 @@ def test_errors(gpu_ctx):
         abi.Volume(gpu_ctx, (8, 8, 1), (1, 1, 1), "u8")
     with pytest.raises(abi.RvkError):
         abi.render_drr(gpu_ctx, v, view, abi.MarchParams(0.0, 0.0, 1, 1, 1), d, out, frame=1)
+
+
+# ---------------------------------------------------------------------------
+# the compositing benchmark's sample count: the steps-taken side output of a
+# plain march (skip rule off) summed over pixels
+@pytest.mark.parametrize("which", ["c1", "c3"])
+def test_plain_march_sample_count(gpu_ctx, which):
+    s = configs.scene_c1(scale=0.25) if which == "c1" else _scene_c3_small()
+    march = dict(s.march, skip_rule=0)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    vd = configs.view_dict(s.views[0])
+    if which == "c1":
+        _, (depth, _, _) = H.gpu_1d(gpu_ctx, vh, s.tf1d, s.views[0], march, with_aux=True)
+        n64 = orc.count_samples(s.volume, s.spacing, vd, march, tf1d=s.tf1d)
+    else:
+        gmax = s.tf2d["gradmag_max"]
+        vh.compute_gradmag(0, gmax)
+        _, (depth, _, _) = H.gpu_2d(gpu_ctx, vh, s.tf2d, s.views[0], march, use_gradmag=True, with_aux=True)
+        gm = H.gradmag_u8(s.volume, s.spacing, gmax)
+        tf = dict(s.tf2d, gradmag=gm, gradmag_scale=float(np.float32(gmax) / np.float32(255.0)))
+        n64 = orc.count_samples(s.volume, s.spacing, vd, march, tf2d=tf)
+    vh.close()
+    n_gpu = int(depth.astype(np.float64).sum())
+    print(f"{which}: plain-march samples GPU {n_gpu}, FP64 oracle {n64}")
+    assert n64 > 0 and abs(n_gpu - n64) <= 1e-3 * n64

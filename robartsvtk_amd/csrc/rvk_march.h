@@ -95,7 +95,8 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
   const bool ref_mode = a.output == RVK_DRR_RGBA8_REF;
   const float lg = logf(256.0f * a.offset);
   float acc = ref_mode ? lg : 0.0f;
-  const float stop_at = ref_mode ? 0.0f : -lg;
+  // termination at acc <= stop_at only with ref_term (otherwise never)
+  const float stop_at = !a.ref_term ? -__builtin_huge_valf() : (ref_mode ? 0.0f : -lg);
   const float j = ray_offset(a.jitter, x, y);
   int k = step_count(r.num_steps - j);
   V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
@@ -129,7 +130,7 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
         if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
         else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
         if (!sa) { add(alpha); visit(q, t, alpha, gr); }
-        if (a.ref_term && acc <= stop_at) break;
+        if (acc <= stop_at) break;
       } else {
         if (!sa) { step(p, inc); k--; }
         sb = !sa;
@@ -143,7 +144,7 @@ __device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const Vol
       if (alpha < 0.0f && !in_key) {
         add(alpha);
         visit(q, t, alpha, gr);
-        if (a.ref_term && acc <= stop_at) break;
+        if (acc <= stop_at) break;
       }
     }
   }

@@ -165,31 +165,10 @@ template <> struct CellLoad<float> : CellLoad32<float> {};
 template <> struct CellLoad<int32_t> : CellLoad32<int32_t> {};
 template <> struct CellLoad<uint32_t> : CellLoad32<uint32_t> {};
 
-#ifndef RVK_CELL_POLY
-#define RVK_CELL_POLY 0
-#endif
-// Bytes of one stored cell of voxel type VT.
-template <typename VT> __host__ __device__ constexpr size_t cell_bytes() {
-  return RVK_CELL_POLY ? 32 : 8 * sizeof(VT);
-}
-
-// Trilinear polynomial of a cell: f(x,y,z) = a + b x + c y + d z + e x y +
-// f x z + g y z + h x y z on [0,1]^3, from its 8 corners (order dx+2dy+4dz).
-__host__ __device__ __forceinline__ void cell_poly(const float v[8], float c[8]) {
-  const float b = v[1] - v[0], cc = v[2] - v[0], d = v[4] - v[0];
-  const float e = (v[3] - v[2]) - b, f = (v[5] - v[4]) - b, g = (v[6] - v[4]) - cc;
-  const float h = ((v[7] - v[6]) - (v[5] - v[4])) - e;
-  c[0] = v[0]; c[1] = b; c[2] = cc; c[3] = d; c[4] = e; c[5] = f; c[6] = g; c[7] = h;
-}
-
 // Voxel (i,j,k) (in range) = corner 000 of cell (i+1, j+1, k+1).
 template <typename VT>
 __device__ __forceinline__ float load_voxel(const VT* __restrict__ d, const VolGeom& g, int i, int j, int k) {
-#if RVK_CELL_POLY
-  return reinterpret_cast<const float*>(d)[(size_t)cell_index(g, i + 1, j + 1, k + 1) * 8];
-#else
   return (float)d[(size_t)cell_index(g, i + 1, j + 1, k + 1) * 8];
-#endif
 }
 
 // Cell and weights of tex3D(unnormalised, linear, clamp) at p, given the
@@ -222,39 +201,6 @@ __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float hx, float hy
 // a + t (b - a) as one rounding of the product-sum
 __device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
 
-#if RVK_CELL_POLY
-__device__ __forceinline__ void load_poly(const void* d, uint32_t cell, float k[8]) {
-  const float4* p = reinterpret_cast<const float4*>(d) + (size_t)cell * 2;
-  const float4 a = p[0], b = p[1];
-  k[0] = a.x; k[1] = a.y; k[2] = a.z; k[3] = a.w; k[4] = b.x; k[5] = b.y; k[6] = b.z; k[7] = b.w;
-}
-template <typename VT>
-__device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGeom& g, float hx, float hy, float hz) {
-  const CellPos c = cell_pos(g, hx, hy, hz);
-  float k[8];
-  load_poly(d, c.cell, k);
-  const float u1 = fmaf(k[7], c.az, k[4]), u2 = fmaf(k[5], c.az, k[1]);
-  const float u3 = fmaf(u1, c.ay, u2);
-  const float u4 = fmaf(k[6], c.az, k[2]), u5 = fmaf(k[3], c.az, k[0]);
-  const float u6 = fmaf(u4, c.ay, u5);
-  return fmaf(u3, c.ax, u6);
-}
-template <typename VT>
-__device__ __forceinline__ float trilinear_vg(const VT* __restrict__ d, const VolGeom& g, float hx, float hy, float hz,
-                                              V3<float>& gr) {
-  const CellPos c = cell_pos(g, hx, hy, hz);
-  float k[8];
-  load_poly(d, c.cell, k);
-  const float u1 = fmaf(k[7], c.az, k[4]), u2 = fmaf(k[5], c.az, k[1]);
-  const float u3 = fmaf(u1, c.ay, u2);
-  const float u4 = fmaf(k[6], c.az, k[2]), u5 = fmaf(k[3], c.az, k[0]);
-  const float u6 = fmaf(u4, c.ay, u5);
-  gr.x = u3;
-  gr.y = fmaf(u1, c.ax, u4);
-  gr.z = fmaf(fmaf(k[7], c.ay, k[5]), c.ax, fmaf(k[6], c.ay, k[3]));
-  return fmaf(u3, c.ax, u6);
-}
-#else
 // Trilinear value at the shifted coordinate h = p - 0.5.
 template <typename VT>
 __device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGeom& g, float hx, float hy, float hz) {
@@ -289,7 +235,6 @@ __device__ __forceinline__ float trilinear_vg(const VT* __restrict__ d, const Vo
   gr.z = c.zs ? c1 - c0 : 0.0f;
   return lerp(c0, c1, c.az);
 }
-#endif
 
 // tex1D(normalised, linear, clamp) on an n-entry table.
 __device__ __forceinline__ float tf1d_lookup(const float* t, int n, float u) {

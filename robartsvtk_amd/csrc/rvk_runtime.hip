@@ -147,15 +147,6 @@ __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict
     const int k = min(max(cz - 1 + (q >> 2), 0), nz - 1);
     v[q] = src[(long long)i + (long long)j * nx + (long long)k * nx * ny];
   }
-#if RVK_CELL_POLY
-  float fv[8], k[8];
-  for (int q = 0; q < 8; ++q) fv[q] = (float)v[q];
-  cell_poly(fv, k);
-  float4* o4 = reinterpret_cast<float4*>(dst) + c * 2;
-  o4[0] = make_float4(k[0], k[1], k[2], k[3]);
-  o4[1] = make_float4(k[4], k[5], k[6], k[7]);
-  return;
-#endif
   VT* out = dst + c * 8;
   if constexpr (sizeof(VT) == 4) {
     reinterpret_cast<uint4*>(out)[0] = *reinterpret_cast<const uint4*>(&v[0]);
@@ -395,7 +386,7 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
   RVK_HIP(hipSetDevice(v->ctx->device));
   const size_t es = dtype_size(v->dtype);
   const size_t n = (size_t)v->dims[0] * v->dims[1] * v->dims[2];
-  if (!v->frames[frame]) RVK_HIP(hipMalloc(&v->frames[frame], v->n_cells * (RVK_CELL_POLY ? 32 : 8 * es)));
+  if (!v->frames[frame]) RVK_HIP(hipMalloc(&v->frames[frame], v->n_cells * 8 * es));
   const void* dsrc = src;
   void* staging = nullptr;
   if (!src_on_device) {
@@ -422,7 +413,7 @@ int rvk_volume_compute_gradmag(rvk_volume* v, int frame, float max_gradient) {
   if (!(max_gradient > 0.0f) || !std::isfinite(max_gradient))
     return fail(RVK_ERR_INVALID_ARGUMENT, "max_gradient must be positive and finite");
   RVK_HIP(hipSetDevice(v->ctx->device));
-  if (!v->gradmag[frame]) RVK_HIP(hipMalloc((void**)&v->gradmag[frame], v->n_cells * (RVK_CELL_POLY ? 32 : 8)));
+  if (!v->gradmag[frame]) RVK_HIP(hipMalloc((void**)&v->gradmag[frame], v->n_cells * 8));
   v->gradmag_scale[frame] = max_gradient / 255.0f;
   const float inv = 255.0f / max_gradient;
   uint8_t* linear = nullptr;

@@ -175,6 +175,15 @@ int rvk_volume_set_frame(rvk_volume* vol, int frame, const void* src, int src_on
  * the device by central differences (0.5/spacing), scale = max_gradient/255. */
 int rvk_volume_compute_gradmag(rvk_volume* vol, int frame, float max_gradient);
 int rvk_volume_destroy(rvk_volume* vol);
+/* Cell order of the device layout (no reference counterpart: the reference
+ * keeps a float cudaArray whose tiling the driver picks).  Each trilinear
+ * 2x2x2 footprint is stored whole; RVK_CELLS_PLAIN orders the cells x-fastest,
+ * RVK_CELLS_BRICK4 groups them in 4x4x4 bricks.  The default is BRICK4 when
+ * the cells take more than 1 GiB, PLAIN below; set_layout must come before
+ * the first rvk_volume_set_frame.  Renders give identical results in both. */
+enum { RVK_CELLS_PLAIN = 0, RVK_CELLS_BRICK4 = 1 };
+int rvk_volume_set_layout(rvk_volume* vol, int layout);
+int rvk_volume_get_layout(const rvk_volume* vol, int* layout);
 int rvk_volume_dims(const rvk_volume* vol, int32_t dims[3], double spacing[3], int* dtype, int* n_frames);
 
 /* ---- transfer functions -----------------------------------------------------

@@ -95,6 +95,8 @@ _SIGS = {
     "rvk_volume_set_frame": (_I, [_P, _I, _P, _I]),
     "rvk_volume_compute_gradmag": (_I, [_P, _I, C.c_float]),
     "rvk_volume_destroy": (_I, [_P]),
+    "rvk_volume_set_layout": (_I, [_P, _I]),
+    "rvk_volume_get_layout": (_I, [_P, C.POINTER(_I)]),
     "rvk_volume_dims": (_I, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(_I)]),
     "rvk_tf1d_create": (_I, [_P, _I, _P, _P, _P, _P, _P, C.POINTER(Tf1dParams), C.POINTER(_P)]),
     "rvk_tf1d_destroy": (_I, [_P]),
@@ -223,7 +225,7 @@ class Context:
 
 
 class Volume:
-    """rvk_volume: native-type, bricked frames on the device."""
+    """rvk_volume: native-type cell volumes (one per frame) on the device."""
 
     def __init__(self, ctx: Context, dims, spacing, dtype, n_frames: int = 1):
         self.ctx = ctx
@@ -236,10 +238,13 @@ class Volume:
         self.handle = h
 
     @classmethod
-    def from_array(cls, ctx: Context, arr: np.ndarray, spacing=(1.0, 1.0, 1.0)) -> "Volume":
-        """arr is indexed [z, y, x] (x fastest in memory, VTK point order)."""
+    def from_array(cls, ctx: Context, arr: np.ndarray, spacing=(1.0, 1.0, 1.0), layout: int | None = None) -> "Volume":
+        """arr is indexed [z, y, x] (x fastest in memory, VTK point order);
+        layout None keeps the library's default cell order."""
         arr = np.ascontiguousarray(arr)
         v = cls(ctx, (arr.shape[2], arr.shape[1], arr.shape[0]), spacing, arr.dtype)
+        if layout is not None:
+            v.set_layout(layout)
         v.set_frame(0, arr)
         return v
 
@@ -251,6 +256,15 @@ class Volume:
             check(lib().rvk_volume_set_frame(self.handle, frame, data.ctypes.data_as(C.c_void_p), 0))
         else:  # torch tensor on the device
             check(lib().rvk_volume_set_frame(self.handle, frame, C.c_void_p(data.data_ptr()), 1))
+
+    def set_layout(self, layout: int) -> None:
+        """CELLS_PLAIN or CELLS_BRICK4; before the first set_frame."""
+        check(lib().rvk_volume_set_layout(self.handle, int(layout)))
+
+    def layout(self) -> int:
+        out = C.c_int()
+        check(lib().rvk_volume_get_layout(self.handle, C.byref(out)))
+        return out.value
 
     def compute_gradmag(self, frame: int, max_gradient: float) -> None:
         check(lib().rvk_volume_compute_gradmag(self.handle, frame, float(max_gradient)))
@@ -350,6 +364,7 @@ def count_ray_steps(ctx: Context, vol: Volume, view: View, march: MarchParams) -
 
 
 SCHED_GRID, SCHED_PERSISTENT = 1, 2
+CELLS_PLAIN, CELLS_BRICK4 = 0, 1
 
 
 class DrrPlan:

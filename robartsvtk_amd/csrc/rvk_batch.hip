@@ -77,7 +77,7 @@ __device__ __forceinline__ void item_pixel(int tiles16_x, int local, int lane, i
 // One pixel of one view.  With JAC the march also gathers trilinear gradients
 // and the pixel's pose Jacobian dL/dpose_i = A . dstart_i + B . dinc_i is
 // written next to the image (the registration step's single march).
-template <typename VT, bool SKIP, bool PRECISE, bool JAC>
+template <typename CV, bool SKIP, bool PRECISE, bool JAC>
 __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int lane) {
   const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
   const BatchView& bv = a.views[v];
@@ -87,14 +87,14 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
   const Ray r = form_ray(a.g, bv.v, a.step_len, x, y);
   const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
   if (!JAC) {
-    a.out[pi] = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a.g, a.m, r, x, y);
+    a.out[pi] = drr_march<CV, SKIP, PRECISE>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
     return;
   }
   const float ds = sqrtf(r.inc.x * r.inc.x * a.g.sp[0] * a.g.sp[0] + r.inc.y * r.inc.y * a.g.sp[1] * a.g.sp[1] +
                          r.inc.z * r.inc.z * a.g.sp[2] * a.g.sp[2]);
   V3<float> A{0, 0, 0}, B{0, 0, 0};
   PoseVisit visit{ds * a.m.slope, ray_offset(a.m.jitter, x, y), A, B};
-  a.out[pi] = drr_march<VT, SKIP, true, true>((const VT*)a.vol, a.g, a.m, r, x, y, visit);
+  a.out[pi] = drr_march<CV, SKIP, true, true>(cells_of<CV>(a.vol), a.g, a.m, r, x, y, visit);
   const bool any = A.x != 0.0f || A.y != 0.0f || A.z != 0.0f || B.x != 0.0f || B.y != 0.0f || B.z != 0.0f;
 #pragma unroll 1
   for (int i = 0; i < 6; ++i) {
@@ -109,19 +109,19 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
 }
 
 // One wave per tile; workgroups XCD-remapped so neighbouring tiles share an L2.
-template <typename VT, bool SKIP, bool PRECISE, bool JAC>
+template <typename CV, bool SKIP, bool PRECISE, bool JAC>
 __global__ __launch_bounds__(256) void batch_grid_kernel(BatchArgs a) {
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int item = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (item >= a.n_items) return;
-  forward_item<VT, SKIP, PRECISE, JAC>(a, item, threadIdx.x & 63);
+  forward_item<CV, SKIP, PRECISE, JAC>(a, item, threadIdx.x & 63);
 }
 
 // Persistent: the tile range is cut into kNumXcd contiguous queues.  A wave
 // drains the queue of the XCD its workgroup was dispatched to (blockIdx mod
 // kNumXcd under round-robin dispatch), then the others in turn; every queue
 // is finite, so every wave reaches the end.
-template <typename VT, bool SKIP, bool PRECISE, bool JAC>
+template <typename CV, bool SKIP, bool PRECISE, bool JAC>
 __global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int home = blockIdx.x % kNumXcd;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a) {
       w = __builtin_amdgcn_readfirstlane(w);
       const int item = begin + w;
       if (item >= end) break;
-      forward_item<VT, SKIP, PRECISE, JAC>(a, item, lane);
+      forward_item<CV, SKIP, PRECISE, JAC>(a, item, lane);
     }
   }
 }
@@ -260,7 +260,7 @@ __device__ __forceinline__ float dloss_at(const float* dldi, const double* stats
   return dldi ? dldi[pi] : dloss_dimage(stats + 8 * v, img[pi], tgt[pi]);
 }
 
-template <typename VT, bool SKIP>
+template <typename CV, bool SKIP>
 __global__ __launch_bounds__(256) void adjoint_pose_kernel(AdjArgs a) {
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int item = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void adjoint_pose_kernel(AdjArgs a) {
     V3<float> A{0, 0, 0}, B{0, 0, 0};
     const float jt = ray_offset(a.b.m.jitter, x, y);
     PoseVisit visit{ds * a.b.m.slope, jt, A, B};
-    const float I = drr_march<VT, SKIP, true, true>((const VT*)a.b.vol, g, a.b.m, r, x, y, visit);
+    const float I = drr_march<CV, SKIP, true, true>(cells_of<CV>(a.b.vol), g, a.b.m, r, x, y, visit);
     const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
     const float dLdI = dloss_at(a.dldi, a.stats, a.img, a.tgt, v, pi);
     // dI/dL = -I for exp(-L) output, 1 for L output
@@ -412,6 +412,7 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
 
 __device__ __forceinline__ float hat(float t) { return fmaxf(0.0f, 1.0f - fabsf(t)); }
 
+template <int L>
 __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
   const VolGeom& g = a.b.g;
   const long long vid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
   const int i = (int)(vid % g.n[0]);
   const int jj = (int)((vid / g.n[0]) % g.n[1]);
   const int k = (int)(vid / ((long long)g.n[0] * g.n[1]));
-  const float* vol = (const float*)a.b.vol;
+  const Cells<float, L> vol = cells_of<Cells<float, L>>(a.b.vol);
   // With mu = value (slope > 0, intercept 0) a sample is accumulated exactly
   // when its trilinear value is > 0.  Samples in this voxel's support read
   // cells inside [i-1, i+1]^3, so if every value there is <= 0 none is.
@@ -497,20 +498,6 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
     }
   }
   a.grad[vid] = acc;
-}
-
-// ---------------------------------------------------------------------------
-template <typename F> static int dispatch_vt_b(int dtype, F&& f) {
-  switch (dtype) {
-    case RVK_U8: return f((uint8_t*)nullptr);
-    case RVK_I8: return f((int8_t*)nullptr);
-    case RVK_U16: return f((uint16_t*)nullptr);
-    case RVK_I16: return f((int16_t*)nullptr);
-    case RVK_U32: return f((uint32_t*)nullptr);
-    case RVK_I32: return f((int32_t*)nullptr);
-    case RVK_F32: return f((float*)nullptr);
-    default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
-  }
 }
 
 }  // namespace rvk
@@ -618,19 +605,19 @@ BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
 int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s, bool jac = false) {
   BatchArgs a = batch_args(p, out);
   const bool skip = p->march.skip_rule != 0, pr = p->march.precise != 0;
-  return dispatch_vt_b(p->vol->dtype, [&](auto tag) {
-    using VT = std::remove_pointer_t<decltype(tag)>;
+  return dispatch_cells(p->vol->dtype, p->vol->layout, [&](auto tag) {
+    using CV = decltype(tag);
     if (p->schedule == RVK_SCHED_PERSISTENT) {
       hipLaunchKernelGGL(reset_queues_kernel, dim3(1), dim3(64), 0, s, p->d_queues);
       const int blocks = std::max(1, std::min(p->n_cu * p->waves_per_simd, (p->n_items + 3) / 4));
-#define RVK_LP(S, P, J) hipLaunchKernelGGL((batch_persistent_kernel<VT, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
+#define RVK_LP(S, P, J) hipLaunchKernelGGL((batch_persistent_kernel<CV, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
       if (jac) { if (skip) RVK_LP(true, true, true); else RVK_LP(false, true, true); }
       else if (skip) { if (pr) RVK_LP(true, true, false); else RVK_LP(true, false, false); }
       else { if (pr) RVK_LP(false, true, false); else RVK_LP(false, false, false); }
 #undef RVK_LP
     } else {
       const int blocks = (p->n_items + 3) / 4;
-#define RVK_LG(S, P, J) hipLaunchKernelGGL((batch_grid_kernel<VT, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
+#define RVK_LG(S, P, J) hipLaunchKernelGGL((batch_grid_kernel<CV, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
       if (jac) { if (skip) RVK_LG(true, true, true); else RVK_LG(false, true, true); }
       else if (skip) { if (pr) RVK_LG(true, true, false); else RVK_LG(true, false, false); }
       else { if (pr) RVK_LG(false, true, false); else RVK_LG(false, false, false); }
@@ -927,10 +914,10 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     aa.stats = nullptr;
     aa.tgt = nullptr;
     aa.part = p->d_pose_part;
-    int rc = dispatch_vt_b(p->vol->dtype, [&](auto tag) {
-      using VT = std::remove_pointer_t<decltype(tag)>;
-      if (p->march.skip_rule) hipLaunchKernelGGL((adjoint_pose_kernel<VT, true>), dim3(blocks), dim3(256), 0, s, aa);
-      else hipLaunchKernelGGL((adjoint_pose_kernel<VT, false>), dim3(blocks), dim3(256), 0, s, aa);
+    int rc = dispatch_cells(p->vol->dtype, p->vol->layout, [&](auto tag) {
+      using CV = decltype(tag);
+      if (p->march.skip_rule) hipLaunchKernelGGL((adjoint_pose_kernel<CV, true>), dim3(blocks), dim3(256), 0, s, aa);
+      else hipLaunchKernelGGL((adjoint_pose_kernel<CV, false>), dim3(blocks), dim3(256), 0, s, aa);
       return (int)RVK_OK;
     });
     if (rc) return rc;
@@ -952,7 +939,12 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     va.grad = grad_vol;
     hipLaunchKernelGGL(ray_record_kernel, dim3(blocks), dim3(256), 0, s, va);
     const long long nvox = (long long)p->vol->dims[0] * p->vol->dims[1] * p->vol->dims[2];
-    hipLaunchKernelGGL(volume_grad_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, va);
+    if (p->vol->layout == kCellLayoutBrick4)
+      hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutBrick4>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
+                         s, va);
+    else
+      hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPlain>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s,
+                         va);
     RVK_HIP(hipGetLastError());
   }
   return RVK_OK;

@@ -29,14 +29,14 @@ struct DrrArgs {
   int tiles_x;
 };
 
-template <typename VT, bool SKIP, bool PRECISE>
+template <typename CV, bool SKIP, bool PRECISE>
 __global__ __launch_bounds__(256) void drr_kernel(DrrArgs a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int x, y;
   tile_pixel(tile, a.tiles_x, x, y);
   if (x >= a.v.W || y >= a.v.H) return;
   const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
-  const float v = drr_march<VT, SKIP, PRECISE>((const VT*)a.vol, a.g, a.m, r, x, y);
+  const float v = drr_march<CV, SKIP, PRECISE>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
   const long idx = (long)y * a.v.W + x;
   if (a.m.output == RVK_DRR_RGBA8_REF) {
     // V/...DRR...cuh:163-166
@@ -74,7 +74,7 @@ struct Cast1dArgs {
 // change the result: diffuse = specular = 0, specular power >= 0 and no
 // gradient-opacity table (non-finite multiplier); then shadeD = ambient and
 // shadeS = 0 exactly, whatever the Phong-Lambert factor.
-template <typename VT, bool SKIP, bool GRAD, bool PRECISE>
+template <typename CV, bool SKIP, bool GRAD, bool PRECISE>
 __global__ __launch_bounds__(256) void cast1d_kernel(Cast1dArgs a) {
   extern __shared__ float lds[];
   const int n = a.n;
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void cast1d_kernel(Cast1dArgs a) {
   tile_pixel(tile, a.tiles_x, x, y);
   if (x >= a.v.W || y >= a.v.H) return;
   const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
-  const VT* __restrict__ vol = (const VT*)a.vol;
+  const CV vol = cells_of<CV>(a.vol);
   const VolGeom& g = a.g;
 
   float Tr = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
@@ -192,6 +192,9 @@ struct Cast2dArgs {
   rvk_tf2d_params p;
   const Tf2dCell* cells;
   const Tf2dCell* kcells;
+  const uint32_t* occ;   // opacity bitmaps staged in LDS, or NULL
+  const uint32_t* kocc;
+  int occ_words;
   const void* vol;
   const uint8_t* gradmag;
   float gradmag_scale;
@@ -205,14 +208,25 @@ struct Cast2dArgs {
 // V/...2D...cuh:30-351.  DUAL selects the keyhole-TF variant (WithDual).
 // GMAG reads |grad| from the precomputed u8 volume instead of 6 extra gathers;
 // without a direction the Phong-Lambert factor is taken as 1.
-template <typename VT, bool SKIP, bool DUAL, bool GMAG, bool PRECISE>
+// With an opacity bitmap in LDS the empty test (alpha > 0 is false) needs no
+// table fetch; the cell is read only for samples that contribute.
+template <typename CV, bool SKIP, bool DUAL, bool GMAG, bool PRECISE>
 __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
+  extern __shared__ uint32_t s_occ[];
+  const bool use_occ = a.occ != nullptr;
+  if (use_occ) {
+    const int w = a.occ_words;
+    for (int i = threadIdx.x; i < w; i += 256) s_occ[i] = a.occ[i];
+    if (DUAL)
+      for (int i = threadIdx.x; i < w; i += 256) s_occ[w + i] = a.kocc[i];
+    __syncthreads();
+  }
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int x, y;
   tile_pixel(tile, a.tiles_x, x, y);
   if (x >= a.v.W || y >= a.v.H) return;
   const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
-  const VT* __restrict__ vol = (const VT*)a.vol;
+  const CV vol = cells_of<CV>(a.vol);
   const VolGeom& g = a.g;
   const int n = a.n;
 
@@ -249,7 +263,7 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
     V3<float> gr{0.0f, 0.0f, 0.0f};
     float gidx, gmag;
     if (GMAG) {
-      gmag = trilinear(a.gradmag, g, q.x, q.y, q.z) * a.gradmag_scale;
+      gmag = trilinear(Cells<uint8_t, CV::kLog2>{a.gradmag}, g, q.x, q.y, q.z) * a.gradmag_scale;
       gidx = gmult * (log2f(gmag * gmag + goff) + glo);
     } else {
       gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
@@ -261,7 +275,13 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
     }
     const int ti = tf2d_index(n, u, gidx);
     const Tf2dCell* __restrict__ cells = (DUAL && in_key) ? a.kcells : a.cells;
-    const float4 c0 = *(const float4*)&cells[ti].alpha;
+    float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (use_occ) {
+      const uint32_t* occ = s_occ + ((DUAL && in_key) ? a.occ_words : 0);
+      if ((occ[ti >> 5] >> (ti & 31)) & 1u) c0 = *(const float4*)&cells[ti].alpha;
+    } else {
+      c0 = *(const float4*)&cells[ti].alpha;
+    }
     const float alpha = c0.x;
     auto accumulate = [&]() {
       const float4 c1 = *(const float4*)&cells[ti].spec_power;
@@ -355,19 +375,6 @@ __global__ __launch_bounds__(256) void count_steps_kernel(VolGeom g, ViewGeom v,
 }
 
 // ===========================================================================
-template <typename F> static int dispatch_vt(int dtype, F&& f) {
-  switch (dtype) {
-    case RVK_U8: return f((uint8_t*)nullptr);
-    case RVK_I8: return f((int8_t*)nullptr);
-    case RVK_U16: return f((uint16_t*)nullptr);
-    case RVK_I16: return f((int16_t*)nullptr);
-    case RVK_U32: return f((uint32_t*)nullptr);
-    case RVK_I32: return f((int32_t*)nullptr);
-    case RVK_F32: return f((float*)nullptr);
-    default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
-  }
-}
-
 static int check_march(const rvk_march_params* m) {
   if (!m) return fail(RVK_ERR_INVALID_ARGUMENT, "march params are NULL");
   if (!(m->step_scale > 0.0f) || !std::isfinite(m->step_scale))
@@ -412,13 +419,13 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
   a.vol = vol->frames[frame];
   a.out = out;
   const int nblk = launch_grid(view, a.tiles_x);
-  return dispatch_vt(vol->dtype, [&](auto tag) {
-    using VT = std::remove_pointer_t<decltype(tag)>;
+  return dispatch_cells(vol->dtype, vol->layout, [&](auto tag) {
+    using CV = decltype(tag);
     const bool pr = march->precise != 0, skip = march->skip_rule != 0;
-    if (skip && pr) hipLaunchKernelGGL((drr_kernel<VT, true, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
-    else if (skip) hipLaunchKernelGGL((drr_kernel<VT, true, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
-    else if (pr) hipLaunchKernelGGL((drr_kernel<VT, false, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
-    else hipLaunchKernelGGL((drr_kernel<VT, false, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    if (skip && pr) hipLaunchKernelGGL((drr_kernel<CV, true, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else if (skip) hipLaunchKernelGGL((drr_kernel<CV, true, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else if (pr) hipLaunchKernelGGL((drr_kernel<CV, false, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((drr_kernel<CV, false, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);
     RVK_HIP(hipGetLastError());
     return (int)RVK_OK;
   });
@@ -451,10 +458,10 @@ int rvk_render_1d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf1d
   const bool grad = !(tf->p.diffuse == 0.0f && tf->p.specular == 0.0f && tf->p.specular_power >= 0.0f &&
                       !std::isfinite(tf->p.gradient_multiplier));
   const bool skip = march->skip_rule != 0, pr = march->precise != 0;
-  return dispatch_vt(vol->dtype, [&](auto tag) {
-    using VT = std::remove_pointer_t<decltype(tag)>;
+  return dispatch_cells(vol->dtype, vol->layout, [&](auto tag) {
+    using CV = decltype(tag);
 #define RVK_L1D(S, G, P) \
-  hipLaunchKernelGGL((cast1d_kernel<VT, S, G, P>), dim3(nblk), dim3(256), lds, ctx->stream, a)
+  hipLaunchKernelGGL((cast1d_kernel<CV, S, G, P>), dim3(nblk), dim3(256), lds, ctx->stream, a)
     if (pr) {
       if (skip) { if (grad) RVK_L1D(true, true, true); else RVK_L1D(true, false, true); }
       else { if (grad) RVK_L1D(false, true, true); else RVK_L1D(false, false, true); }
@@ -488,6 +495,12 @@ int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d
   a.p = tf->p;
   a.cells = tf->cells;
   a.kcells = tf->kcells;
+  a.occ = tf->occ;
+  a.kocc = tf->kocc;
+  a.occ_words = tf->occ_words;
+  // the keyhole bitmap is needed exactly when the keyhole TF is
+  if (tf->kcells && !tf->kocc) a.occ = nullptr;
+  const size_t lds = a.occ ? (size_t)a.occ_words * sizeof(uint32_t) * (tf->kcells ? 2 : 1) : 0;
   a.vol = vol->frames[frame];
   a.gradmag = use_gradmag ? vol->gradmag[frame] : nullptr;
   a.gradmag_scale = use_gradmag ? vol->gradmag_scale[frame] : 0.0f;
@@ -498,12 +511,12 @@ int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d
   const int nblk = launch_grid(view, a.tiles_x);
   const bool skip = march->skip_rule != 0, dual = tf->kcells != nullptr, gm = use_gradmag != 0;
   const bool pr = march->precise != 0;
-  return dispatch_vt(vol->dtype, [&](auto tag) {
-    using VT = std::remove_pointer_t<decltype(tag)>;
+  return dispatch_cells(vol->dtype, vol->layout, [&](auto tag) {
+    using CV = decltype(tag);
 #define RVK_L2D(S, D, G)                                                                              \
   do {                                                                                                \
-    if (pr) hipLaunchKernelGGL((cast2d_kernel<VT, S, D, G, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);  \
-    else hipLaunchKernelGGL((cast2d_kernel<VT, S, D, G, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);    \
+    if (pr) hipLaunchKernelGGL((cast2d_kernel<CV, S, D, G, true>), dim3(nblk), dim3(256), lds, ctx->stream, a);  \
+    else hipLaunchKernelGGL((cast2d_kernel<CV, S, D, G, false>), dim3(nblk), dim3(256), lds, ctx->stream, a);    \
   } while (0)
     if (skip) {
       if (dual) { if (gm) RVK_L2D(true, true, true); else RVK_L2D(true, true, false); }

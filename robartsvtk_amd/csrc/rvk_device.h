@@ -79,33 +79,45 @@ __device__ __forceinline__ uint32_t to_u8(float v) {
 // the order dx + 2 dy + 4 dz, as one 8/16/32-byte vector (u8/16-bit/32-bit
 // voxels).  Cell indices run over [0, n+1] per axis (the sample coordinate is
 // clamped to [-1, n] first), so edge clamping is baked into the cells and a
-// sample costs one vector load and no index clamps.  The volume grows 8x; the
-// cells are grouped in bricks of (2^RVK_CELL_BRICK_LOG2)^3 with separable
-// addressing; on the c2 DRR (256^3 i16, 1024^2) plain x-fastest cell order
-// (edge 1) measured 759 G samples/s against 667 (edge 2) and 681 (edge 4).
-#ifndef RVK_CELL_BRICK_LOG2
-#define RVK_CELL_BRICK_LOG2 0
-#endif
-constexpr int kCellBrickLog2 = RVK_CELL_BRICK_LOG2;
-constexpr int kCellBrickEdge = 1 << kCellBrickLog2;
-constexpr int kCellBrickCells = kCellBrickEdge * kCellBrickEdge * kCellBrickEdge;
+// sample costs one vector load and no index clamps.  The volume grows 8x.
+//
+// Cell order: x-fastest (brick edge 1) or 4x4x4-cell bricks, chosen per
+// volume; both are kernel template instances.  Plain order suits rays that
+// run along the x rows and dense sampling (c2: 0.83 ms against 0.92 with
+// bricks); bricks suit large volumes seen from arbitrary directions (c3:
+// 6.6 ms against 8.2).
+template <int LOG2> struct CellOrder {
+  static constexpr int kLog2 = LOG2;
+  static constexpr int kEdge = 1 << LOG2;
+  static constexpr int kCells = kEdge * kEdge * kEdge;
+  static __device__ __forceinline__ uint32_t x(int c) {
+    return (uint32_t)(((c >> LOG2) * kCells) | (c & (kEdge - 1)));
+  }
+  // The row and slab terms use the full-rate 24-bit multiply: the slab
+  // stride is < 2^24 (checked when a volume is created), the brick row/slab
+  // index < 2^13, and the slab term stays below the cell count (< 2^32), so
+  // the low 32 bits of the 24-bit products are exact.
+  static __device__ __forceinline__ uint32_t y(int c, uint32_t cys) {
+    return __umul24((uint32_t)(c >> LOG2), cys) + (uint32_t)((c & (kEdge - 1)) << LOG2);
+  }
+  static __device__ __forceinline__ uint32_t z(int c, uint32_t czs) {
+    return __umul24((uint32_t)(c >> LOG2), czs) + (uint32_t)((c & (kEdge - 1)) << (2 * LOG2));
+  }
+  static __device__ __forceinline__ uint32_t index(const VolGeom& g, int cx, int cy, int cz) {
+    return x(cx) + y(cy, g.cys) + z(cz, g.czs);
+  }
+};
+constexpr int kCellLayoutPlain = 0;
+constexpr int kCellLayoutBrick4 = 2;
 
-__device__ __forceinline__ uint32_t cell_x(int c) {
-  return (uint32_t)(((c >> kCellBrickLog2) * kCellBrickCells) | (c & (kCellBrickEdge - 1)));
-}
-// The row term stays below the slab stride czs <= 2^24 (checked when a volume
-// is created), so it takes the full-rate 24-bit multiply.
-__device__ __forceinline__ uint32_t cell_y(int c, uint32_t cys) {
-  return __umul24((uint32_t)(c >> kCellBrickLog2), cys) + (uint32_t)((c & (kCellBrickEdge - 1)) << kCellBrickLog2);
-}
-// The slab term stays below the cell count (< 2^32) with czs <= 2^24 and the
-// slab index < 2^13, so the low 32 bits of the 24-bit product are exact.
-__device__ __forceinline__ uint32_t cell_z(int c, uint32_t czs) {
-  return __umul24((uint32_t)(c >> kCellBrickLog2), czs) + (uint32_t)((c & (kCellBrickEdge - 1)) << (2 * kCellBrickLog2));
-}
-__device__ __forceinline__ uint32_t cell_index(const VolGeom& g, int cx, int cy, int cz) {
-  return cell_x(cx) + cell_y(cy, g.cys) + cell_z(cz, g.czs);
-}
+// A cell volume of voxel type T in cell order LOG2 (what the kernels are
+// templated on; `d` is the device array of cells).
+template <typename T, int LOG2> struct Cells {
+  using value_type = T;
+  using order = CellOrder<LOG2>;
+  static constexpr int kLog2 = LOG2;
+  const T* __restrict__ d;
+};
 
 // The 8 corner values of a cell as floats, in the order dx + 2 dy + 4 dz.
 template <typename VT> struct CellLoad;
@@ -165,10 +177,14 @@ template <> struct CellLoad<float> : CellLoad32<float> {};
 template <> struct CellLoad<int32_t> : CellLoad32<int32_t> {};
 template <> struct CellLoad<uint32_t> : CellLoad32<uint32_t> {};
 
+template <typename CV> __device__ __forceinline__ CV cells_of(const void* p) {
+  return CV{static_cast<const typename CV::value_type*>(p)};
+}
+
 // Voxel (i,j,k) (in range) = corner 000 of cell (i+1, j+1, k+1).
-template <typename VT>
-__device__ __forceinline__ float load_voxel(const VT* __restrict__ d, const VolGeom& g, int i, int j, int k) {
-  return (float)d[(size_t)cell_index(g, i + 1, j + 1, k + 1) * 8];
+template <typename T, int L>
+__device__ __forceinline__ float load_voxel(Cells<T, L> v, const VolGeom& g, int i, int j, int k) {
+  return (float)v.d[(size_t)CellOrder<L>::index(g, i + 1, j + 1, k + 1) * 8];
 }
 
 // Cell and weights of tex3D(unnormalised, linear, clamp) at p, given the
@@ -181,6 +197,7 @@ struct CellPos {
   float ax, ay, az;
   bool xs, ys, zs;  // the two corners along the axis are distinct voxels
 };
+template <int L>
 __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float hx, float hy, float hz) {
   const float x = __builtin_amdgcn_fmed3f(hx, -1.0f, (float)g.n[0]);
   const float y = __builtin_amdgcn_fmed3f(hy, -1.0f, (float)g.n[1]);
@@ -191,7 +208,7 @@ __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float hx, float hy
   c.ay = y - fy;
   c.az = z - fz;
   const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-  c.cell = cell_index(g, ix + 1, iy + 1, iz + 1);
+  c.cell = CellOrder<L>::index(g, ix + 1, iy + 1, iz + 1);
   c.xs = ix >= 0 && ix < g.n[0] - 1;
   c.ys = iy >= 0 && iy < g.n[1] - 1;
   c.zs = iz >= 0 && iz < g.n[2] - 1;
@@ -202,11 +219,11 @@ __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float hx, float hy
 __device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
 
 // Trilinear value at the shifted coordinate h = p - 0.5.
-template <typename VT>
-__device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGeom& g, float hx, float hy, float hz) {
-  const CellPos c = cell_pos(g, hx, hy, hz);
+template <typename T, int L>
+__device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, float hx, float hy, float hz) {
+  const CellPos c = cell_pos<L>(g, hx, hy, hz);
   float v[8];
-  CellLoad<VT>::get(d, c.cell, v);
+  CellLoad<T>::get(vol.d, c.cell, v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
   const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
@@ -220,12 +237,12 @@ __device__ __forceinline__ float trilinear(const VT* __restrict__ d, const VolGe
 // d/dx = lerp_z(lerp_y(v100-v000, v110-v010), lerp_y(v101-v001, v111-v011)),
 // d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0; axes whose two corners
 // clamp to one voxel get a zero derivative.
-template <typename VT>
-__device__ __forceinline__ float trilinear_vg(const VT* __restrict__ d, const VolGeom& g, float hx, float hy, float hz,
+template <typename T, int L>
+__device__ __forceinline__ float trilinear_vg(Cells<T, L> vol, const VolGeom& g, float hx, float hy, float hz,
                                               V3<float>& gr) {
-  const CellPos c = cell_pos(g, hx, hy, hz);
+  const CellPos c = cell_pos<L>(g, hx, hy, hz);
   float v[8];
-  CellLoad<VT>::get(d, c.cell, v);
+  CellLoad<T>::get(vol.d, c.cell, v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
   const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);

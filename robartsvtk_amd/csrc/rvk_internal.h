@@ -22,6 +22,7 @@ struct rvk_volume {
   double spacing[3] = {1, 1, 1};
   int dtype = RVK_U8;
   int n_frames = 0;
+  int layout = 0;                // cell order: kCellLayoutPlain or kCellLayoutBrick4
   int ncb[3] = {0, 0, 0};        // cell-brick grid: ceil((dims+2)/brick edge)
   size_t n_cells = 0;            // cells incl. brick padding (each 8 voxels)
   void* frames[RVK_MAX_FRAMES] = {};    // cell volumes, n_cells * 8 voxels
@@ -47,8 +48,17 @@ struct rvk_tf2d {
   int n = 0;
   Tf2dCell* cells = nullptr;   // n*n
   Tf2dCell* kcells = nullptr;  // keyhole TF or null
+  // Opacity bitmaps (bit c = alpha[c] > 0), staged in LDS by the 2D caster
+  // when they fit (n <= kTf2dOccMaxN); words = ceil(n*n / 32) each.
+  uint32_t* occ = nullptr;
+  uint32_t* kocc = nullptr;
+  int occ_words = 0;
   rvk_tf2d_params p{};
 };
+
+// Largest 2D TF whose opacity bitmap the caster stages in LDS (8 KiB; 16 KiB
+// with the keyhole TF).
+constexpr int kTf2dOccMaxN = 256;
 
 namespace rvk {
 
@@ -62,6 +72,24 @@ int hip_fail(hipError_t e, const char* what);
   } while (0)
 
 size_t dtype_size(int dtype);
+
+// Calls f(Cells<T, L>{nullptr}) for the volume's voxel type and cell order.
+template <int L, typename F> int dispatch_cells_t(int dtype, F&& f) {
+  switch (dtype) {
+    case RVK_U8: return f(Cells<uint8_t, L>{nullptr});
+    case RVK_I8: return f(Cells<int8_t, L>{nullptr});
+    case RVK_U16: return f(Cells<uint16_t, L>{nullptr});
+    case RVK_I16: return f(Cells<int16_t, L>{nullptr});
+    case RVK_U32: return f(Cells<uint32_t, L>{nullptr});
+    case RVK_I32: return f(Cells<int32_t, L>{nullptr});
+    case RVK_F32: return f(Cells<float, L>{nullptr});
+    default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
+  }
+}
+template <typename F> int dispatch_cells(int dtype, int layout, F&& f) {
+  if (layout == kCellLayoutBrick4) return dispatch_cells_t<kCellLayoutBrick4>(dtype, f);
+  return dispatch_cells_t<kCellLayoutPlain>(dtype, f);
+}
 VolGeom vol_geom(const rvk_volume* v);
 ViewGeom view_geom(const rvk_view* v);
 float min_spacing(const rvk_volume* v);

@@ -89,8 +89,8 @@ struct NoVisit {
 // coordinate (h = p0 - 0.5 + t*inc in PRECISE mode); with VG the trilinear
 // gradient at h comes from the same gathers as the value (the adjoint marches
 // through this very control flow).
-template <typename VT, bool SKIP, bool PRECISE, bool VG = false, typename F = NoVisit>
-__device__ __forceinline__ float drr_march(const VT* __restrict__ vol, const VolGeom& g, const DrrMarchArgs& a,
+template <typename CV, bool SKIP, bool PRECISE, bool VG = false, typename F = NoVisit>
+__device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMarchArgs& a,
                                            const Ray& r, int x, int y, F&& visit = F()) {
   const bool ref_mode = a.output == RVK_DRR_RGBA8_REF;
   const float lg = logf(256.0f * a.offset);

@@ -58,8 +58,9 @@ VolGeom vol_geom(const rvk_volume* v) {
     g.bounds[2 * a] = 0.0f;
     g.bounds[2 * a + 1] = (float)v->dims[a] - 1.0f;
   }
-  g.cys = (uint32_t)(kCellBrickCells * v->ncb[0]);
-  g.czs = (uint32_t)(kCellBrickCells * v->ncb[0] * v->ncb[1]);
+  const uint32_t bc = 1u << (3 * v->layout);  // cells per brick
+  g.cys = bc * (uint32_t)v->ncb[0];
+  g.czs = bc * (uint32_t)v->ncb[0] * (uint32_t)v->ncb[1];
   return g;
 }
 
@@ -125,20 +126,21 @@ template <> struct CellVec<int8_t> { using T = uint2; };
 template <> struct CellVec<uint16_t> { using T = uint4; };
 template <> struct CellVec<int16_t> { using T = uint4; };
 
-template <typename VT>
+template <typename VT, int L>
 __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst, int nx,
                                                             int ny, int nz, int ncbx, int ncby, long long n_cells) {
+  using O = CellOrder<L>;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cells) return;
-  const long long b = c / kCellBrickCells;
-  const int o = (int)(c % kCellBrickCells);
+  const long long b = c / O::kCells;
+  const int o = (int)(c % O::kCells);
   const int bx = (int)(b % ncbx);
   const int by = (int)((b / ncbx) % ncby);
   const int bz = (int)(b / ((long long)ncbx * ncby));
-  const int m = kCellBrickEdge - 1;
-  const int cx = bx * kCellBrickEdge + (o & m);
-  const int cy = by * kCellBrickEdge + ((o >> kCellBrickLog2) & m);
-  const int cz = bz * kCellBrickEdge + (o >> (2 * kCellBrickLog2));
+  const int m = O::kEdge - 1;
+  const int cx = bx * O::kEdge + (o & m);
+  const int cy = by * O::kEdge + ((o >> L) & m);
+  const int cz = bz * O::kEdge + (o >> (2 * L));
   alignas(16) VT v[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -160,9 +162,9 @@ __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict
 // reference's on-the-fly gradient takes there (tex3D at p +- 1 with clamp,
 // times 0.5/spacing; V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:263-271),
 // quantised to u8 as round(|g| / scale), written x-fastest.
-template <typename VT>
-__global__ __launch_bounds__(256) void gradmag_kernel(const VT* __restrict__ vol, uint8_t* __restrict__ out,
-                                                      VolGeom g, float inv_scale, long long total) {
+template <typename CV>
+__global__ __launch_bounds__(256) void gradmag_kernel(CV vol, uint8_t* __restrict__ out, VolGeom g, float inv_scale,
+                                                      long long total) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total) return;
   const int i = (int)(e % g.n[0]);
@@ -181,37 +183,48 @@ __global__ __launch_bounds__(256) void gradmag_kernel(const VT* __restrict__ vol
   out[e] = (uint8_t)fminf(rintf(q), 255.0f);
 }
 
-template <typename VT>
+template <typename VT, int L>
 static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
   const long long total = (long long)v->n_cells;
   const int blocks = (int)((total + 255) / 256);
-  hipLaunchKernelGGL(cell_relayout_kernel<VT>, dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc,
+  hipLaunchKernelGGL((cell_relayout_kernel<VT, L>), dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc,
                      (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
   RVK_HIP(hipGetLastError());
   return RVK_OK;
 }
 
-template <typename VT>
+template <typename CV>
 static int launch_gradmag(rvk_volume* v, int frame, float inv_scale, uint8_t* linear) {
+  using VT = typename CV::value_type;
   const long long total = (long long)v->dims[0] * v->dims[1] * v->dims[2];
   const int blocks = (int)((total + 255) / 256);
-  hipLaunchKernelGGL(gradmag_kernel<VT>, dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)v->frames[frame],
-                     linear, vol_geom(v), inv_scale, total);
+  hipLaunchKernelGGL((gradmag_kernel<CV>), dim3(blocks), dim3(256), 0, v->ctx->stream,
+                     CV{(const VT*)v->frames[frame]}, linear, vol_geom(v), inv_scale, total);
   RVK_HIP(hipGetLastError());
-  return launch_relayout<uint8_t>(v, linear, v->gradmag[frame]);
+  return launch_relayout<uint8_t, CV::kLog2>(v, linear, v->gradmag[frame]);
 }
 
-template <typename F> static int dispatch_dtype(int dtype, F&& f) {
-  switch (dtype) {
-    case RVK_U8: return f((uint8_t*)nullptr);
-    case RVK_I8: return f((int8_t*)nullptr);
-    case RVK_U16: return f((uint16_t*)nullptr);
-    case RVK_I16: return f((int16_t*)nullptr);
-    case RVK_U32: return f((uint32_t*)nullptr);
-    case RVK_I32: return f((int32_t*)nullptr);
-    case RVK_F32: return f((float*)nullptr);
-    default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
-  }
+// Cell grid and count of a volume in its cell order; fails when the cell
+// addressing limits (slab < 2^24 cells, < 2^32 cells) are exceeded.
+static int size_cells(rvk_volume* v) {
+  const int edge = 1 << v->layout;
+  for (int a = 0; a < 3; ++a) v->ncb[a] = (v->dims[a] + 2 + edge - 1) / edge;
+  const size_t bc = (size_t)1 << (3 * v->layout);
+  v->n_cells = bc * v->ncb[0] * v->ncb[1] * v->ncb[2];
+  const size_t slab = bc * v->ncb[0] * v->ncb[1];
+  if (slab >= ((size_t)1 << 24))  // cell addressing multiplies with 24-bit operands
+    return fail(RVK_ERR_INVALID_ARGUMENT, "volume x/y extent too large: %zu cells per z slab, limit 2^24 - 1", slab);
+  if (v->n_cells > (size_t)0xffffffffu)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "volume too large: %zu cells exceed 32-bit cell indices", v->n_cells);
+  return RVK_OK;
+}
+
+// Default cell order: bricks once the cell volume passes 1 GiB (large
+// volumes, where the plain order's long strides across rows and slabs miss
+// the caches), plain x-fastest order below.
+static int default_layout(const int32_t dims[3], int dtype) {
+  const double bytes = 8.0 * (double)dtype_size(dtype) * (dims[0] + 2.0) * (dims[1] + 2.0) * (dims[2] + 2.0);
+  return bytes > (double)(1ull << 30) ? kCellLayoutBrick4 : kCellLayoutPlain;
 }
 
 }  // namespace rvk
@@ -363,19 +376,36 @@ int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[
   for (int a = 0; a < 3; ++a) {
     v->dims[a] = dims[a];
     v->spacing[a] = spacing[a];
-    v->ncb[a] = (dims[a] + 2 + kCellBrickEdge - 1) / kCellBrickEdge;
   }
-  v->n_cells = (size_t)kCellBrickCells * v->ncb[0] * v->ncb[1] * v->ncb[2];
-  const size_t slab = (size_t)kCellBrickCells * v->ncb[0] * v->ncb[1];
-  if (slab >= ((size_t)1 << 24)) {  // cell addressing multiplies with 24-bit operands
+  v->layout = default_layout(dims, dtype);
+  const int rc = size_cells(v);
+  if (rc != RVK_OK) {
     delete v;
-    return fail(RVK_ERR_INVALID_ARGUMENT, "volume x/y extent too large: %zu cells per z slab, limit 2^24 - 1", slab);
-  }
-  if (v->n_cells > (size_t)0xffffffffu) {
-    delete v;
-    return fail(RVK_ERR_INVALID_ARGUMENT, "volume too large: %zu cells exceed 32-bit cell indices", v->n_cells);
+    return rc;
   }
   *out = v;
+  return RVK_OK;
+}
+
+int rvk_volume_set_layout(rvk_volume* v, int layout) {
+  if (!v) return fail(RVK_ERR_INVALID_ARGUMENT, "volume is NULL");
+  if (layout != RVK_CELLS_PLAIN && layout != RVK_CELLS_BRICK4)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown cell layout %d", layout);
+  for (int f = 0; f < RVK_MAX_FRAMES; ++f)
+    if (v->frames[f]) return fail(RVK_ERR_INVALID_ARGUMENT, "the cell layout is fixed once a frame has data");
+  const int old = v->layout;
+  v->layout = layout == RVK_CELLS_BRICK4 ? kCellLayoutBrick4 : kCellLayoutPlain;
+  const int rc = size_cells(v);
+  if (rc != RVK_OK) {
+    v->layout = old;
+    size_cells(v);
+  }
+  return rc;
+}
+
+int rvk_volume_get_layout(const rvk_volume* v, int* layout) {
+  if (!v || !layout) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *layout = v->layout == kCellLayoutBrick4 ? RVK_CELLS_BRICK4 : RVK_CELLS_PLAIN;
   return RVK_OK;
 }
 
@@ -395,9 +425,9 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
     if (e != hipSuccess) { hipFree(staging); return hip_fail(e, "hipMemcpyAsync(volume)"); }
     dsrc = staging;
   }
-  int rc = dispatch_dtype(v->dtype, [&](auto tag) {
-    using VT = std::remove_pointer_t<decltype(tag)>;
-    return launch_relayout<VT>(v, dsrc, v->frames[frame]);
+  int rc = dispatch_cells(v->dtype, v->layout, [&](auto tag) {
+    using CV = decltype(tag);
+    return launch_relayout<typename CV::value_type, CV::kLog2>(v, dsrc, v->frames[frame]);
   });
   if (staging) {
     hipError_t e = hipStreamSynchronize(v->ctx->stream);
@@ -418,10 +448,7 @@ int rvk_volume_compute_gradmag(rvk_volume* v, int frame, float max_gradient) {
   const float inv = 255.0f / max_gradient;
   uint8_t* linear = nullptr;
   RVK_HIP(hipMalloc((void**)&linear, (size_t)v->dims[0] * v->dims[1] * v->dims[2]));
-  rc = dispatch_dtype(v->dtype, [&](auto tag) {
-    using VT = std::remove_pointer_t<decltype(tag)>;
-    return launch_gradmag<VT>(v, frame, inv, linear);
-  });
+  rc = dispatch_cells(v->dtype, v->layout, [&](auto tag) { return launch_gradmag<decltype(tag)>(v, frame, inv, linear); });
   const hipError_t e = hipStreamSynchronize(v->ctx->stream);
   hipFree(linear);
   if (rc == RVK_OK && e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
@@ -485,6 +512,17 @@ int rvk_tf1d_destroy(rvk_tf1d* t) {
   return RVK_OK;
 }
 
+static int upload_occupancy(int n, const float* alpha, uint32_t** dst) {
+  const size_t nn = (size_t)n * n, words = (nn + 31) / 32;
+  std::vector<uint32_t> bits(words, 0u);
+  for (size_t c = 0; c < nn; ++c)
+    if (alpha[c] > 0.0f) bits[c >> 5] |= 1u << (c & 31);
+  RVK_HIP(hipMalloc((void**)dst, words * sizeof(uint32_t)));
+  hipError_t e = hipMemcpy(*dst, bits.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { hipFree(*dst); *dst = nullptr; return hip_fail(e, "hipMemcpy(tf2d occupancy)"); }
+  return RVK_OK;
+}
+
 static int upload_cells(int n, const float* const tables[8], Tf2dCell** dst) {
   for (int i = 0; i < 8; ++i)
     if (!tables[i]) return fail(RVK_ERR_INVALID_ARGUMENT, "2D TF table %d is NULL", i);
@@ -518,8 +556,15 @@ int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const flo
   t->p = *p;
   int rc = upload_cells(n, tables, &t->cells);
   if (rc == RVK_OK && ktables) rc = upload_cells(n, ktables, &t->kcells);
+  if (rc == RVK_OK && n <= kTf2dOccMaxN) {
+    t->occ_words = (int)(((size_t)n * n + 31) / 32);
+    rc = upload_occupancy(n, tables[0], &t->occ);
+    if (rc == RVK_OK && ktables) rc = upload_occupancy(n, ktables[0], &t->kocc);
+  }
   if (rc != RVK_OK) {
     if (t->cells) hipFree(t->cells);
+    if (t->kcells) hipFree(t->kcells);
+    if (t->occ) hipFree(t->occ);
     delete t;
     return rc;
   }
@@ -533,6 +578,8 @@ int rvk_tf2d_destroy(rvk_tf2d* t) {
   hipStreamSynchronize(t->ctx->stream);
   if (t->cells) hipFree(t->cells);
   if (t->kcells) hipFree(t->kcells);
+  if (t->occ) hipFree(t->occ);
+  if (t->kocc) hipFree(t->kocc);
   delete t;
   return RVK_OK;
 }

@@ -255,3 +255,27 @@ def test_volume_gradient_vs_fp64_fd(gpu_ctx):
     print(f"directional derivative: analytic {an:.6e}, fd {fd:.6e}")
     assert abs(an - fd) <= 1e-3 * abs(fd)
     plan.close()
+
+
+def test_register_and_volume_gradient_identical_across_cell_layouts(gpu_ctx):
+    torch = _torch()
+    s, tg = _registration_scene(n_views=2, scale=0.1)
+    tgt = torch.from_numpy(np.concatenate([t.ravel() for t in tg])).cuda()
+    res = []
+    for layout in (abi.CELLS_PLAIN, abi.CELLS_BRICK4):
+        vol = abi.Volume.from_array(gpu_ctx, s.volume, s.spacing, layout=layout)
+        plan = _plan_with_derivatives(gpu_ctx, s, vol)
+        out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+        loss = torch.empty(plan.n_views, dtype=torch.float32, device="cuda")
+        grad = torch.empty(plan.n_views * 6, dtype=torch.float32, device="cuda")
+        plan.register(out, tgt, loss, grad)
+        dldi = torch.empty_like(out)
+        plan.ncc(out, tgt, loss, dldi)
+        gvol = torch.zeros(s.volume.size, dtype=torch.float32, device="cuda")
+        plan.backward(out, dldi, grad_volume=gvol)
+        torch.cuda.synchronize()
+        res.append((out.cpu(), loss.cpu(), grad.cpu(), gvol.cpu()))
+        plan.close()
+        vol.close()
+    for a, b in zip(*res):
+        assert torch.equal(a, b)

@@ -340,3 +340,45 @@ def test_plain_march_sample_count(gpu_ctx, which):
     n_gpu = int(depth.astype(np.float64).sum())
     print(f"{which}: plain-march samples GPU {n_gpu}, FP64 oracle {n64}")
     assert n64 > 0 and abs(n_gpu - n64) <= 1e-3 * n64
+
+
+# ---------------------------------------------------------------------------
+# cell order: plain x-fastest and 4x4x4 bricks give identical renders
+def test_cell_layouts_render_identically(gpu_ctx):
+    import torch
+    outs = {}
+    for layout in (abi.CELLS_PLAIN, abi.CELLS_BRICK4):
+        s2 = configs.scene_c2(scale=0.25)
+        v2 = abi.Volume.from_array(gpu_ctx, s2.volume, s2.spacing, layout=layout)
+        assert v2.layout() == layout
+        d = H.gpu_drr(gpu_ctx, v2, s2.views[0], s2.march, s2.drr)
+        s1 = configs.scene_c1(scale=0.25)
+        v1 = abi.Volume.from_array(gpu_ctx, s1.volume, s1.spacing, layout=layout)
+        c1 = H.gpu_1d(gpu_ctx, v1, s1.tf1d, s1.views[0], dict(s1.march, term_T=1.0 / 64))
+        s3 = _scene_c3_small()
+        v3 = abi.Volume.from_array(gpu_ctx, s3.volume, s3.spacing, layout=layout)
+        v3.compute_gradmag(0, s3.tf2d["gradmag_max"])
+        c3 = H.gpu_2d(gpu_ctx, v3, s3.tf2d, s3.views[0], s3.march, use_gradmag=True)
+        c3b = H.gpu_2d(gpu_ctx, v3, s3.tf2d, s3.views[0], s3.march, use_gradmag=False)
+        outs[layout] = (d, c1, c3, c3b)
+        for v in (v1, v2, v3):
+            v.close()
+    for a, b in zip(outs[abi.CELLS_PLAIN], outs[abi.CELLS_BRICK4]):
+        assert np.array_equal(a, b)
+    assert outs[abi.CELLS_PLAIN][2][..., 3].max() > 100
+
+
+def test_cell_layout_rules(gpu_ctx):
+    small = abi.Volume(gpu_ctx, (64, 64, 64), (1.0, 1.0, 1.0), np.int16)
+    assert small.layout() == abi.CELLS_PLAIN          # 35 MB of cells
+    big = abi.Volume(gpu_ctx, (512, 512, 512), (1.0, 1.0, 1.0), np.uint16)
+    assert big.layout() == abi.CELLS_BRICK4           # 2.2 GB of cells
+    big.set_layout(abi.CELLS_PLAIN)                   # allowed before data
+    assert big.layout() == abi.CELLS_PLAIN
+    with pytest.raises(abi.RvkError):
+        big.set_layout(7)
+    small.set_frame(0, np.zeros((64, 64, 64), np.int16))
+    with pytest.raises(abi.RvkError):
+        small.set_layout(abi.CELLS_BRICK4)           # fixed once a frame has data
+    small.close()
+    big.close()

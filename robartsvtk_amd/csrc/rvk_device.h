@@ -120,10 +120,15 @@ template <typename T, int LOG2> struct Cells {
 };
 
 // The 8 corner values of a cell as floats, in the order dx + 2 dy + 4 dz.
+// A cell's raw bits (`Raw`, what one vector load returns) and their 8 corner
+// values as floats (`unpack`), so a march can load a cell ahead of its use.
 template <typename VT> struct CellLoad;
 template <> struct CellLoad<uint8_t> {
-  static __device__ __forceinline__ void get(const uint8_t* __restrict__ d, uint32_t c, float v[8]) {
-    const uint2 w = *reinterpret_cast<const uint2*>(d + (size_t)c * 8);
+  using Raw = uint2;
+  static __device__ __forceinline__ Raw raw(const uint8_t* __restrict__ d, uint32_t c) {
+    return *reinterpret_cast<const uint2*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       v[i] = (float)((w.x >> (8 * i)) & 0xffu);
@@ -132,8 +137,11 @@ template <> struct CellLoad<uint8_t> {
   }
 };
 template <> struct CellLoad<int8_t> {
-  static __device__ __forceinline__ void get(const int8_t* __restrict__ d, uint32_t c, float v[8]) {
-    const int2 w = *reinterpret_cast<const int2*>(d + (size_t)c * 8);
+  using Raw = int2;
+  static __device__ __forceinline__ Raw raw(const int8_t* __restrict__ d, uint32_t c) {
+    return *reinterpret_cast<const int2*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       v[i] = (float)((int)((uint32_t)w.x << (24 - 8 * i)) >> 24);
@@ -142,8 +150,11 @@ template <> struct CellLoad<int8_t> {
   }
 };
 template <> struct CellLoad<uint16_t> {
-  static __device__ __forceinline__ void get(const uint16_t* __restrict__ d, uint32_t c, float v[8]) {
-    const uint4 w = *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
+  using Raw = uint4;
+  static __device__ __forceinline__ Raw raw(const uint16_t* __restrict__ d, uint32_t c) {
+    return *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
     const uint32_t q[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -153,8 +164,11 @@ template <> struct CellLoad<uint16_t> {
   }
 };
 template <> struct CellLoad<int16_t> {
-  static __device__ __forceinline__ void get(const int16_t* __restrict__ d, uint32_t c, float v[8]) {
-    const int4 w = *reinterpret_cast<const int4*>(d + (size_t)c * 8);
+  using Raw = int4;
+  static __device__ __forceinline__ Raw raw(const int16_t* __restrict__ d, uint32_t c) {
+    return *reinterpret_cast<const int4*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
     const int q[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -163,19 +177,22 @@ template <> struct CellLoad<int16_t> {
     }
   }
 };
-template <typename W> struct CellLoad32 {
-  static __device__ __forceinline__ void get(const W* __restrict__ d, uint32_t c, float v[8]) {
-    const W* p = d + (size_t)c * 8;
-    const auto a = *reinterpret_cast<const typename std::conditional<std::is_same<W, float>::value, float4,
-                                     typename std::conditional<std::is_same<W, int32_t>::value, int4, uint4>::type>::type*>(p);
-    const auto b = *(reinterpret_cast<const decltype(a)*>(p) + 1);
-    v[0] = (float)a.x; v[1] = (float)a.y; v[2] = (float)a.z; v[3] = (float)a.w;
-    v[4] = (float)b.x; v[5] = (float)b.y; v[6] = (float)b.z; v[7] = (float)b.w;
+template <typename W, typename V4> struct CellLoad32 {
+  struct Raw {
+    V4 a, b;
+  };
+  static __device__ __forceinline__ Raw raw(const W* __restrict__ d, uint32_t c) {
+    const V4* p = reinterpret_cast<const V4*>(d + (size_t)c * 8);
+    return Raw{p[0], p[1]};
+  }
+  static __device__ __forceinline__ void unpack(const Raw& w, float v[8]) {
+    v[0] = (float)w.a.x; v[1] = (float)w.a.y; v[2] = (float)w.a.z; v[3] = (float)w.a.w;
+    v[4] = (float)w.b.x; v[5] = (float)w.b.y; v[6] = (float)w.b.z; v[7] = (float)w.b.w;
   }
 };
-template <> struct CellLoad<float> : CellLoad32<float> {};
-template <> struct CellLoad<int32_t> : CellLoad32<int32_t> {};
-template <> struct CellLoad<uint32_t> : CellLoad32<uint32_t> {};
+template <> struct CellLoad<float> : CellLoad32<float, float4> {};
+template <> struct CellLoad<int32_t> : CellLoad32<int32_t, int4> {};
+template <> struct CellLoad<uint32_t> : CellLoad32<uint32_t, uint4> {};
 
 template <typename CV> __device__ __forceinline__ CV cells_of(const void* p) {
   return CV{static_cast<const typename CV::value_type*>(p)};
@@ -223,7 +240,7 @@ template <typename T, int L>
 __device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, float hx, float hy, float hz) {
   const CellPos c = cell_pos<L>(g, hx, hy, hz);
   float v[8];
-  CellLoad<T>::get(vol.d, c.cell, v);
+  CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
   const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
@@ -242,7 +259,7 @@ __device__ __forceinline__ float trilinear_vg(Cells<T, L> vol, const VolGeom& g,
                                               V3<float>& gr) {
   const CellPos c = cell_pos<L>(g, hx, hy, hz);
   float v[8];
-  CellLoad<T>::get(vol.d, c.cell, v);
+  CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
   const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);

@@ -382,9 +382,12 @@ static int check_march(const rvk_march_params* m) {
   return RVK_OK;
 }
 
-static int launch_grid(const rvk_view* v, int& tiles_x) {
-  tiles_x = (v->width + 15) / 16;
-  return tiles_x * ((v->height + 15) / 16);
+// Workgroups for a view: whole super-tiles of 16x16-pixel tiles; `stiles_x`
+// receives the number of super-tile columns.
+static int launch_grid(const rvk_view* v, int& stiles_x) {
+  const int e = 16 << kSuperLog2;
+  stiles_x = (v->width + e - 1) / e;
+  return stiles_x * ((v->height + e - 1) / e) << (2 * kSuperLog2);
 }
 
 }  // namespace rvk

@@ -565,11 +565,23 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
-__device__ __forceinline__ void tile_pixel(int tile, int tiles_x, int& px, int& py) {
+// 16x16-pixel workgroup tiles are ordered in super-tiles of 2^kSuperLog2 x
+// 2^kSuperLog2 tiles (row-major over super-tiles, row-major inside), so the
+// tiles an XCD runs at one time cover a compact image region (and volume
+// footprint) rather than a few full-width rows.  `stiles_x` is the number of
+// super-tile columns; tiles past the image edge return at once.
+#ifndef RVK_SUPER_LOG2
+#define RVK_SUPER_LOG2 2
+#endif
+constexpr int kSuperLog2 = RVK_SUPER_LOG2;
+__device__ __forceinline__ void tile_pixel(int tile, int stiles_x, int& px, int& py) {
   const int t = threadIdx.x;
   const int w = t >> 6, l = t & 63;
-  px = (tile % tiles_x) * 16 + (w & 1) * 8 + (l & 7);
-  py = (tile / tiles_x) * 16 + (w >> 1) * 8 + (l >> 3);
+  const int st = tile >> (2 * kSuperLog2), in = tile & ((1 << (2 * kSuperLog2)) - 1);
+  const int tx = (st % stiles_x) * (1 << kSuperLog2) + (in & ((1 << kSuperLog2) - 1));
+  const int ty = (st / stiles_x) * (1 << kSuperLog2) + (in >> kSuperLog2);
+  px = tx * 16 + (w & 1) * 8 + (l & 7);
+  py = ty * 16 + (w >> 1) * 8 + (l >> 3);
 }
 
 }  // namespace rvk

@@ -94,6 +94,11 @@ def lib() -> C.CDLL:
             ("orc_2d_f32", [P, P, P, P, P, P, P, C.c_int], C.c_int),
             ("orc_2d_f64", [P, P, P, P, P, P, P, C.c_int], C.c_int),
             ("orc_post_shade_f32", [P, C.c_int, C.c_int, P, P, P, P], C.c_int),
+            ("orc_post_shade_inex_f32", [P, C.c_int, C.c_int, P, P, P, P], C.c_int),
+            ("orc_inex_f32", [P, P, P, C.c_int, P, P, P, P, P, C.c_int], C.c_int),
+            ("orc_inex_f64", [P, P, P, C.c_int, P, P, P, P, P, C.c_int], C.c_int),
+            ("orc_dual_f32", [P, P, P, P, P, P, P, P, C.c_int], C.c_int),
+            ("orc_dual_f64", [P, P, P, P, P, P, P, P, C.c_int], C.c_int),
             ("orc_count_samples", [P, P, P, P, P, C.c_int], C.c_longlong),
             ("orc_sample_f64", [P, C.c_double, C.c_double, C.c_double], C.c_double),
             ("orc_sample_f32", [P, C.c_float, C.c_float, C.c_float], C.c_float),
@@ -267,6 +272,49 @@ def cast2d(vol: np.ndarray, spacing, tf: dict, view: dict, march: dict, precisio
     return rgba8, rgbaf, depth
 
 
+def cast2d_inex(vol: np.ndarray, spacing, tf: dict, inex: np.ndarray, black_keyhole: bool, view: dict, march: dict,
+                precision: str = "f32", nthreads: int | None = None):
+    """In/Ex-logic 2D caster; tf as for cast2d (no keyhole tables), inex the
+    (n, n) in/ex table.  Returns (rgba8, rgba float, depth)."""
+    v, _a = _vol(vol, spacing)
+    vw, _z = _view(view)
+    t, _k = _tf2d(tf)
+    ie = np.ascontiguousarray(inex, np.float32)
+    m = _march(march)
+    W, H = vw.width, vw.height
+    ft = np.float32 if precision == "f32" else np.float64
+    rgba8 = np.zeros((H, W, 4), np.uint8)
+    rgbaf = np.zeros((H, W, 4), ft)
+    depth = np.zeros((H, W), ft)
+    fn = lib().orc_inex_f32 if precision == "f32" else lib().orc_inex_f64
+    rc = fn(C.byref(v), C.byref(t), ie.ctypes.data, int(black_keyhole), C.byref(vw), C.byref(m), rgba8.ctypes.data,
+            rgbaf.ctypes.data, depth.ctypes.data, nthreads or default_threads())
+    assert rc == 0, rc
+    return rgba8, rgbaf, depth
+
+
+def cast_dual(vol1: np.ndarray, vol2: np.ndarray, spacing, tf: dict, view: dict, march: dict, precision: str = "f32",
+              nthreads: int | None = None):
+    """Dual-image caster over two co-registered volumes; tf["params"] gives
+    intensity_low/multiplier (component 1) and gradient_low/multiplier
+    (component 2).  Returns (rgba8, rgba float, depth)."""
+    v1, _a1 = _vol(vol1, spacing)
+    v2, _a2 = _vol(vol2, spacing)
+    vw, _z = _view(view)
+    t, _k = _tf2d(tf)
+    m = _march(march)
+    W, H = vw.width, vw.height
+    ft = np.float32 if precision == "f32" else np.float64
+    rgba8 = np.zeros((H, W, 4), np.uint8)
+    rgbaf = np.zeros((H, W, 4), ft)
+    depth = np.zeros((H, W), ft)
+    fn = lib().orc_dual_f32 if precision == "f32" else lib().orc_dual_f64
+    rc = fn(C.byref(v1), C.byref(v2), C.byref(t), C.byref(vw), C.byref(m), rgba8.ctypes.data, rgbaf.ctypes.data,
+            depth.ctypes.data, nthreads or default_threads())
+    assert rc == 0, rc
+    return rgba8, rgbaf, depth
+
+
 def form_rays(vol_dims, spacing, view: dict, march: dict, precision: str = "f64"):
     """Returns dict of per-pixel arrays (start, inc [H,W,3], num_steps, exclude_start/end, min/max depth)."""
     dummy = np.zeros((1, 1, 1), np.uint8)
@@ -310,6 +358,19 @@ def post_shade(shade: dict, depth: np.ndarray, min_depth: np.ndarray, max_depth:
     rc = lib().orc_post_shade_f32(C.byref(s), W, H, depth.ctypes.data,
                                   np.ascontiguousarray(min_depth, np.float32).ctypes.data,
                                   np.ascontiguousarray(max_depth, np.float32).ctypes.data, rgba.ctypes.data)
+    assert rc == 0
+    return depth, rgba
+
+
+def post_shade_inex(shade: dict, depth: np.ndarray, min_depth: np.ndarray, max_depth: np.ndarray,
+                    rgba8: np.ndarray):
+    s = OrcShade(*[float(shade[k]) for k in ("celr", "cela", "celc", "disr", "disa", "disc")])
+    depth = np.ascontiguousarray(depth, np.float32).copy()
+    rgba = np.ascontiguousarray(rgba8, np.uint8).copy()
+    H, W = depth.shape
+    rc = lib().orc_post_shade_inex_f32(C.byref(s), W, H, depth.ctypes.data,
+                                       np.ascontiguousarray(min_depth, np.float32).ctypes.data,
+                                       np.ascontiguousarray(max_depth, np.float32).ctypes.data, rgba.ctypes.data)
     assert rc == 0
     return depth, rgba
 

@@ -784,6 +784,269 @@ int cast2d_impl(const orc_volume* vol, const orc_tf2d* tf, const orc_view* view,
   });
 }
 
+// ---------------------------------------------------------------------------
+// In/Ex-logic 2D caster: V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.cuh:24-179 (march) and
+// :241-302 (composite).  Differences from the 2D caster: the step count is
+// floor(numSteps) (the offset moves the start only), a sample contributes when
+// inEx > 0 or alpha > 0, the first accumulated sample outside the keyhole
+// range [ee, es] with inEx > 0.5 ends the ray with nothing accumulated, the
+// output is saturated, and with the black keyhole a ray that accumulated is
+// opaque.
+template <typename T, typename VT>
+RGBAT<T> inex_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_tf2d* tf, const float* inex, int black,
+                  const orc_march* mp, const Ray<T>& r, int x, int y, Ctl* ctl) {
+  Ctl none;
+  if (!ctl) ctl = &none;
+  T Tr = T(1), cr = T(0), cg = T(0), cb = T(0);
+  const T lo = (T)tf->intensity_low, mult = (T)tf->intensity_mult;
+  const T glo = (T)tf->gradient_low, gmult = (T)tf->gradient_mult, goff = (T)tf->gradient_offset;
+  const T term = (T)mp->term_T;
+  const T j = jitter_of<T>(mp, x, y);
+  V3<T> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  int k = step_count(r.num_steps);
+  int es = k - fl2i_ru(r.exclude_start);
+  int ee = k - fl2i_rd(r.exclude_end);
+  ctl->ints(k, es, ee);
+  const T depth0 = j + (T)k;
+  const int kst = k;
+  const V3<T>& inc = r.inc;
+  const T sx = vi.sp[0], sy = vi.sp[1], sz = vi.sp[2];
+  const T ray_len = std::sqrt(inc.x * inc.x * sx * sx + inc.y * inc.y * sy * sy + inc.z * inc.z * sz * sz);
+  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+  auto cur = [&]() -> V3<T> {
+    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+    const T t = (T)(kst - k);
+    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
+  };
+  bool skip = false, back = false, special = false;
+  while (k > 0) {
+    if (ctl->count) ++*ctl->count;
+    const V3<T> q = cur();
+    const T u = mult * (trilinear<T>(g, q.x, q.y, q.z) - lo);
+    V3<T> gr;
+    gr.x = (trilinear<T>(g, q.x + T(1), q.y, q.z) - trilinear<T>(g, q.x - T(1), q.y, q.z)) * vi.srecip[0];
+    gr.y = (trilinear<T>(g, q.x, q.y + T(1), q.z) - trilinear<T>(g, q.x, q.y - T(1), q.z)) * vi.srecip[1];
+    gr.z = (trilinear<T>(g, q.x, q.y, q.z + T(1)) - trilinear<T>(g, q.x, q.y, q.z - T(1))) * vi.srecip[2];
+    const T gm2 = gr.x * gr.x + gr.y * gr.y + gr.z * gr.z;
+    const T gidx = gmult * (std::log2(gm2 + goff) + glo);
+    const T gmag = std::sqrt(gm2);
+    const int ti = tf2d_index<T>(tf->n, u, gidx);
+    const T ie = (T)inex[ti], alpha = (T)tf->alpha[ti];
+    const T pl = sat(std::fabs(gr.x * inc.x * sx + gr.y * inc.y * sy + gr.z * inc.z * sz) / (gmag * ray_len));
+    const T shd = (T)tf->ambient[ti] + (T)tf->diffuse[ti] * pl;
+    const T shs = (T)tf->specular[ti] * std::pow(pl, (T)tf->spec_power[ti]);
+    // accumulate after the step; false: the ray ends here (excluded)
+    auto accumulate = [&]() -> bool {
+      if (ctl->decide(!special && !(es >= k && ee <= k) && ie > T(0.5))) return false;
+      special = true;
+      const T m = Tr * alpha;
+      Tr *= (T(1) - alpha);
+      cr += m * sat(shd * (T)tf->r[ti] + shs);
+      cg += m * sat(shd * (T)tf->g[ti] + shs);
+      cb += m * sat(shd * (T)tf->b[ti] + shs);
+      return true;
+    };
+    const bool contributes = ctl->decide(ie > T(0) || alpha > T(0));
+    if (mp->skip_rule) {
+      if (contributes) {
+        back = skip;
+        skip = false;
+        if (back) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
+        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
+        if (!back && !accumulate()) break;
+        if (ctl->decide(Tr < term)) { Tr = T(0); break; }
+      } else {
+        if (!back) { p.x += inc.x; p.y += inc.y; p.z += inc.z; k--; }
+        skip = !back;
+        p.x += inc.x; p.y += inc.y; p.z += inc.z; k--;
+        back = false;
+      }
+    } else {
+      p.x += inc.x; p.y += inc.y; p.z += inc.z; k--;
+      if (contributes) {
+        if (!accumulate()) break;
+        if (ctl->decide(Tr < term)) { Tr = T(0); break; }
+      }
+    }
+  }
+  RGBAT<T> o;
+  o.a = (black && special) ? T(1) : T(1) - Tr;
+  o.r = cr;
+  o.g = cg;
+  o.b = cb;
+  o.depth = depth0 - (T)k;
+  return o;
+}
+
+// ---------------------------------------------------------------------------
+// Dual-image caster: V/CUDA_vtkCudaDualImageVolumeMapper_renderAlgo.cuh:33-208 (no keyhole TF),
+// :210-380 (keyhole TF), :383-517 (composites).  The 2D table is indexed by
+// the two components, shading uses the gradient of component 1, and the step
+// count is trunc(floor(numSteps) - offset) (an int minus a float stored back
+// in an int).  The exclusion bounds round up/down in both variants.
+template <typename T> inline int dual_step_count(T num_steps, T j) {
+  if (!(num_steps > T(0) && num_steps < T(16777216))) return 0;
+  const T f = (T)(int)std::floor(num_steps) - j;
+  return (int)f;
+}
+
+template <typename T, typename VT>
+RGBAT<T> dual_ray(const Grid<VT>& g1, const Grid<VT>& g2, const VolInfo<T>& vi, const orc_tf2d* tf,
+                  const orc_march* mp, const Ray<T>& r, int x, int y, Ctl* ctl) {
+  Ctl none;
+  if (!ctl) ctl = &none;
+  T Tr = T(1), cr = T(0), cg = T(0), cb = T(0);
+  const T lo1 = (T)tf->intensity_low, m1 = (T)tf->intensity_mult;
+  const T lo2 = (T)tf->gradient_low, m2 = (T)tf->gradient_mult;
+  const T term = (T)mp->term_T;
+  const bool dual = tf->k_alpha != nullptr;
+  const T j = jitter_of<T>(mp, x, y);
+  V3<T> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  int k = dual_step_count<T>(r.num_steps, j);
+  int es = k - fl2i_ru(r.exclude_start);
+  int ee = k - fl2i_rd(r.exclude_end);
+  ctl->ints(k, es, ee);
+  const int k0 = k, kst = k;
+  const V3<T>& inc = r.inc;
+  const T sx = vi.sp[0], sy = vi.sp[1], sz = vi.sp[2];
+  const T ray_len = std::sqrt(inc.x * inc.x * sx * sx + inc.y * inc.y * sy * sy + inc.z * inc.z * sz * sz);
+  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+  auto cur = [&]() -> V3<T> {
+    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+    const T t = (T)(kst - k);
+    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
+  };
+  int a = 0, b = 0;
+  while (k > 0) {
+    const bool in_key = (es >= k && ee < k);
+    if (!dual && in_key) {
+      p.x += (T)(k - ee) * inc.x; p.y += (T)(k - ee) * inc.y; p.z += (T)(k - ee) * inc.z;
+      k = ee;
+      a = 0;
+      b = 0;
+      continue;
+    }
+    if (ctl->count) ++*ctl->count;
+    const V3<T> q = cur();
+    const T u1 = m1 * (trilinear<T>(g1, q.x, q.y, q.z) - lo1);
+    const T u2 = m2 * (trilinear<T>(g2, q.x, q.y, q.z) - lo2);
+    V3<T> gr;
+    gr.x = (trilinear<T>(g1, q.x + T(1), q.y, q.z) - trilinear<T>(g1, q.x - T(1), q.y, q.z)) * vi.srecip[0];
+    gr.y = (trilinear<T>(g1, q.x, q.y + T(1), q.z) - trilinear<T>(g1, q.x, q.y - T(1), q.z)) * vi.srecip[1];
+    gr.z = (trilinear<T>(g1, q.x, q.y, q.z + T(1)) - trilinear<T>(g1, q.x, q.y, q.z - T(1))) * vi.srecip[2];
+    const T gmag = std::sqrt(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+    const int ti = tf2d_index<T>(tf->n, u1, u2);
+    const bool kt = dual && in_key;
+    const T alpha = (T)(kt ? tf->k_alpha : tf->alpha)[ti];
+    const T pl = sat(std::fabs(gr.x * inc.x * sx + gr.y * inc.y * sy + gr.z * inc.z * sz) / (gmag * ray_len));
+    const T shd = (T)(kt ? tf->k_ambient : tf->ambient)[ti] + (T)(kt ? tf->k_diffuse : tf->diffuse)[ti] * pl;
+    const T shs = (T)(kt ? tf->k_specular : tf->specular)[ti] * std::pow(pl, (T)(kt ? tf->k_spec_power : tf->spec_power)[ti]);
+    auto accumulate = [&]() {
+      const T m = Tr * alpha;
+      Tr *= (T(1) - alpha);
+      cr += m * sat(shd * (T)(kt ? tf->k_r : tf->r)[ti] + shs);
+      cg += m * sat(shd * (T)(kt ? tf->k_g : tf->g)[ti] + shs);
+      cb += m * sat(shd * (T)(kt ? tf->k_b : tf->b)[ti] + shs);
+    };
+    if (mp->skip_rule) {
+      if (ctl->decide(alpha > T(0))) {
+        a = b;
+        b = 0;
+        if (a) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
+        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
+        if (!a) accumulate();
+        if (ctl->decide(Tr < term)) { Tr = T(0); break; }
+      } else {
+        if (!a) { p.x += inc.x; p.y += inc.y; p.z += inc.z; k--; }
+        b = !a;
+        p.x += inc.x; p.y += inc.y; p.z += inc.z; k--;
+        a = 0;
+      }
+    } else {
+      p.x += inc.x; p.y += inc.y; p.z += inc.z; k--;
+      if (ctl->decide(alpha > T(0))) {
+        accumulate();
+        if (ctl->decide(Tr < term)) { Tr = T(0); break; }
+      }
+    }
+  }
+  RGBAT<T> o;
+  o.a = T(1) - Tr;
+  o.r = cr;
+  o.g = cg;
+  o.b = cb;
+  o.depth = (T)(k0 - k);
+  return o;
+}
+
+template <typename T>
+int inex_impl(const orc_volume* vol, const orc_tf2d* tf, const float* inex, int black, const orc_view* view,
+              const orc_march* mp, uint8_t* rgba8, T* rgbaf, T* depth, int nthreads) {
+  if (!vol || !tf || !inex || !view || !mp || tf->n < 1) return 1;
+  const VolInfo<T> vi = vol_info<T>(vol);
+  const T sl = step_length(vi, mp);
+  const VolInfo<float> vi32 = vol_info<float>(vol);
+  const float sl32 = step_length(vi32, mp);
+  const bool shadow = mp->f32_control && sizeof(T) != sizeof(float);
+  return with_grid(vol, [&](auto g) {
+    parallel_rows(view->height, nthreads, [&](int y) {
+      std::vector<uint8_t> dec;
+      for (int x = 0; x < view->width; ++x) {
+        Ctl ctl;
+        if (shadow) {
+          dec.clear();
+          ctl.rec = &dec;
+          inex_ray<float>(g, vi32, tf, inex, black, mp, form_ray(vi32, view, sl32, x, y), x, y, &ctl);
+          ctl.rec = nullptr;
+          ctl.play = &dec;
+        }
+        const Ray<T> r = form_ray(vi, view, sl, x, y);
+        RGBAT<T> o = inex_ray<T>(g, vi, tf, inex, black, mp, r, x, y, &ctl);
+        o.r = sat(o.r);
+        o.g = sat(o.g);
+        o.b = sat(o.b);
+        o.a = sat(o.a);
+        store_rgba(o, (long)y * view->width + x, rgba8, rgbaf, depth);
+      }
+    });
+    return 0;
+  });
+}
+
+template <typename T>
+int dual_impl(const orc_volume* v1, const orc_volume* v2, const orc_tf2d* tf, const orc_view* view,
+              const orc_march* mp, uint8_t* rgba8, T* rgbaf, T* depth, int nthreads) {
+  if (!v1 || !v2 || !tf || !view || !mp || tf->n < 1) return 1;
+  if (v1->dtype != v2->dtype || v1->dims[0] != v2->dims[0] || v1->dims[1] != v2->dims[1] ||
+      v1->dims[2] != v2->dims[2])
+    return 1;
+  const VolInfo<T> vi = vol_info<T>(v1);
+  const T sl = step_length(vi, mp);
+  const VolInfo<float> vi32 = vol_info<float>(v1);
+  const float sl32 = step_length(vi32, mp);
+  const bool shadow = mp->f32_control && sizeof(T) != sizeof(float);
+  return with_grid(v1, [&](auto g1) {
+    using G = decltype(g1);
+    const G g2{(decltype(g1.d))v2->data, v2->dims[0], v2->dims[1], v2->dims[2]};
+    parallel_rows(view->height, nthreads, [&](int y) {
+      std::vector<uint8_t> dec;
+      for (int x = 0; x < view->width; ++x) {
+        Ctl ctl;
+        if (shadow) {
+          dec.clear();
+          ctl.rec = &dec;
+          dual_ray<float>(g1, g2, vi32, tf, mp, form_ray(vi32, view, sl32, x, y), x, y, &ctl);
+          ctl.rec = nullptr;
+          ctl.play = &dec;
+        }
+        const Ray<T> r = form_ray(vi, view, sl, x, y);
+        store_rgba(dual_ray<T>(g1, g2, vi, tf, mp, r, x, y, &ctl), (long)y * view->width + x, rgba8, rgbaf, depth);
+      }
+    });
+    return 0;
+  });
+}
+
 template <typename T>
 int form_rays_impl(const orc_volume* vol, const orc_view* view, const orc_march* mp, T* out9, T* mind, T* maxd) {
   if (!vol || !view || !mp) return 1;
@@ -868,6 +1131,56 @@ int orc_post_shade_f32(const orc_shade* sp, int W, int H, float* depth, const fl
       gm = 1.0f - sp->celr * sat((gm - sp->cela) * sp->celc);
       gm *= 1.0f - sp->disr * sat((d0 - sp->disa) * sp->disc);
       for (int c = 0; c < 3; ++c) rgba8[4 * i + c] = to_u8(gm * (float)src[4 * i + c]);
+    }
+  return 0;
+}
+
+int orc_inex_f32(const orc_volume* vol, const orc_tf2d* tf, const float* inex, int black_keyhole,
+                 const orc_view* view, const orc_march* mp, uint8_t* rgba8, float* rgbaf, float* depth, int nthreads) {
+  return inex_impl<float>(vol, tf, inex, black_keyhole, view, mp, rgba8, rgbaf, depth, nthreads);
+}
+int orc_inex_f64(const orc_volume* vol, const orc_tf2d* tf, const float* inex, int black_keyhole,
+                 const orc_view* view, const orc_march* mp, uint8_t* rgba8, double* rgbaf, double* depth,
+                 int nthreads) {
+  return inex_impl<double>(vol, tf, inex, black_keyhole, view, mp, rgba8, rgbaf, depth, nthreads);
+}
+int orc_dual_f32(const orc_volume* vol1, const orc_volume* vol2, const orc_tf2d* tf, const orc_view* view,
+                 const orc_march* mp, uint8_t* rgba8, float* rgbaf, float* depth, int nthreads) {
+  return dual_impl<float>(vol1, vol2, tf, view, mp, rgba8, rgbaf, depth, nthreads);
+}
+int orc_dual_f64(const orc_volume* vol1, const orc_volume* vol2, const orc_tf2d* tf, const orc_view* view,
+                 const orc_march* mp, uint8_t* rgba8, double* rgbaf, double* depth, int nthreads) {
+  return dual_impl<double>(vol1, vol2, tf, view, mp, rgba8, rgbaf, depth, nthreads);
+}
+
+// The In/Ex-logic cel shade (V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.cuh:182-239) after
+// normBuffer: the squared depth difference (no square root) and, for pixels
+// with zero alpha, alpha = (signed char)(255 - 255 g) stored as an unsigned
+// byte (float-to-char conversion clamps to [-128, 127]).
+int orc_post_shade_inex_f32(const orc_shade* sp, int W, int H, float* depth, const float* min_depth,
+                            const float* max_depth, uint8_t* rgba8) {
+  if (!sp || !depth || !min_depth || !max_depth || !rgba8) return 1;
+  const long n = (long)W * H;
+  for (long i = 0; i < n; ++i) {
+    const float cur = depth[i], mx = max_depth[i], mn = min_depth[i];
+    depth[i] = (mx > 0.0f) ? (cur + mn) / mx : 1.0f;
+  }
+  std::vector<uint8_t> src(rgba8, rgba8 + 4 * n);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const long i = (long)y * W + x;
+      const long iy = (long)std::min(y + 1, H - 1) * W + x;
+      const long ix = (long)y * W + std::min(x + 1, W - 1);
+      const float d0 = depth[i], dy = depth[iy], dx = depth[ix];
+      float gm = (dx - d0) * (dx - d0) + (dy - d0) * (dy - d0);
+      gm = 1.0f - sp->celr * sat((gm - sp->cela) * sp->celc);
+      gm *= 1.0f - sp->disr * sat((d0 - sp->disa) * sp->disc);
+      for (int c = 0; c < 3; ++c) rgba8[4 * i + c] = to_u8(gm * (float)src[4 * i + c]);
+      if (src[4 * i + 3] == 0) {
+        const float v = 255.0f - 255.0f * gm;
+        int t = std::isnan(v) ? 0 : (int)std::fmax(-128.0f, std::fmin(127.0f, std::trunc(v)));
+        rgba8[4 * i + 3] = (uint8_t)(int8_t)t;
+      }
     }
   return 0;
 }

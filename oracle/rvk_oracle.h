@@ -122,6 +122,26 @@ int orc_2d_f32(const orc_volume* vol, const orc_tf2d* tf, const orc_view* view, 
 int orc_2d_f64(const orc_volume* vol, const orc_tf2d* tf, const orc_view* view, const orc_march* mp,
                uint8_t* rgba8, double* rgbaf, double* depth, int nthreads);
 
+/* In/Ex-logic 2D caster (vtkCuda2DInExLogicVolumeMapper): tf as for the 2D
+ * caster (no keyhole tables), inex the n x n in/ex table, black_keyhole the
+ * UseBlackKeyhole flag.  Output saturated. */
+int orc_inex_f32(const orc_volume* vol, const orc_tf2d* tf, const float* inex, int black_keyhole,
+                 const orc_view* view, const orc_march* mp, uint8_t* rgba8, float* rgbaf, float* depth, int nthreads);
+int orc_inex_f64(const orc_volume* vol, const orc_tf2d* tf, const float* inex, int black_keyhole,
+                 const orc_view* view, const orc_march* mp, uint8_t* rgba8, double* rgbaf, double* depth,
+                 int nthreads);
+/* Dual-image caster (vtkCudaDualImageVolumeMapper): two co-registered volumes
+ * (same dims and dtype); the 2D table is indexed by intensity_mult*(v1 -
+ * intensity_low) and gradient_mult*(v2 - gradient_low); keyhole tables
+ * optional; gradmag fields unused. */
+int orc_dual_f32(const orc_volume* vol1, const orc_volume* vol2, const orc_tf2d* tf, const orc_view* view,
+                 const orc_march* mp, uint8_t* rgba8, float* rgbaf, float* depth, int nthreads);
+int orc_dual_f64(const orc_volume* vol1, const orc_volume* vol2, const orc_tf2d* tf, const orc_view* view,
+                 const orc_march* mp, uint8_t* rgba8, double* rgbaf, double* depth, int nthreads);
+/* The In/Ex-logic mapper's post pass (normBuffer then its cel shade). */
+int orc_post_shade_inex_f32(const orc_shade* sp, int W, int H, float* depth, const float* min_depth,
+                            const float* max_depth, uint8_t* rgba8);
+
 /* Post passes: normBuffer then cel/distance shading, in place. */
 int orc_post_shade_f32(const orc_shade* sp, int W, int H, float* depth, const float* min_depth,
                        const float* max_depth, uint8_t* rgba8);

@@ -336,6 +336,215 @@ def test_2d_uniform_medium_closed_form():
     assert f[10, 12, 3] == pytest.approx(1 - T, rel=1e-12)
 
 
+def _uniform_2d_tf(N, a, red=0.6):
+    """Opacity a at gradient index 0 (a uniform volume has zero gradient),
+    red 0.6, ambient 1: every sample composites (a, 0.6 a)."""
+    tables = {k: np.zeros((N, N), np.float32) for k in orc.TF2D_NAMES}
+    tables["alpha"][0, :] = a
+    tables["red"][:] = red
+    tables["ambient"][:] = 1.0
+    tables["specular_power"][:] = 1.0
+    return {"tables": tables, "params": configs.tf2d_params_for(0.0, 4095.0, 0.0, 100.0)}
+
+
+def test_inex_uniform_medium_closed_form():
+    """In/Ex caster with an all-zero in/ex table is the 2D caster with
+    floor(numSteps) samples: T = (1 - a)^34 on the 17-voxel z path at step 0.5,
+    and with jitter j the count stays 34 (the offset moves the start only) while
+    depth = j + 34."""
+    dims, spacing = (16, 18, 20), (1.0, 1.0, 1.0)
+    view, _, _ = _parallel_z_view(dims, spacing, 24, 20, scale=0.25 * dims[1])
+    vd = configs.view_dict(view)
+    vol = np.full(dims[::-1], 1000, np.uint16)
+    a, N = 0.05, 32
+    tf = _uniform_2d_tf(N, a)
+    inex = np.zeros((N, N), np.float32)
+    m = math.floor((dims[2] - 3) / 0.5)
+    T = (1 - float(np.float32(a))) ** m
+    red = float(np.float32(0.6))
+    for jit in (0, 1):
+        march = {"step_scale": 0.5, "term_T": 0.0, "jitter": jit, "skip_rule": 1}
+        _, f, depth = orc.cast2d_inex(vol, spacing, tf, inex, False, vd, march, "f64")
+        jt = orc.jitter_table()
+        for y in range(5, 15):
+            for x in range(7, 17):
+                assert f[y, x, 0] == pytest.approx(red * (1 - T), rel=1e-12)
+                assert f[y, x, 3] == pytest.approx(1 - T, rel=1e-12)
+                j = float(jt[(x % 16) + 16 * (y % 16)]) if jit else 0.0
+                assert depth[y, x] == pytest.approx(m + j, abs=1e-12)
+    # the 1/32 termination of the In/Ex march: opaque after the first
+    # sample with T < 1/32, i.e. ceil(log(1/32) / log(1 - a)) samples
+    march = {"step_scale": 0.5, "term_T": 1.0 / 32, "jitter": 0, "skip_rule": 1}
+    _, f, depth = orc.cast2d_inex(vol, spacing, _uniform_2d_tf(N, 0.25), inex, False, vd, march, "f64")
+    stop = next(i for i in range(1, m + 1) if (1 - 0.25) ** i < 1.0 / 32)
+    assert f[10, 12, 3] == 1.0 and depth[10, 12] == stop
+    assert f[10, 12, 0] == pytest.approx(red * (1 - 0.75 ** stop), rel=1e-12)
+
+
+def test_inex_exclusion_and_black_keyhole():
+    """In/Ex > 0.5 on the first accumulated sample ends the ray with nothing
+    accumulated (no keyhole range covers it); In/Ex in (0, 0.5] with zero
+    opacity passes, marks the ray, and with the black keyhole makes it opaque
+    black.  Rays that miss the volume stay empty either way."""
+    dims, spacing = (16, 18, 20), (1.0, 1.0, 1.0)
+    view, _, _ = _parallel_z_view(dims, spacing, 24, 20, scale=0.25 * dims[1])
+    vd = configs.view_dict(view)
+    vol = np.full(dims[::-1], 1000, np.uint16)
+    N = 32
+    march = {"step_scale": 0.5, "term_T": 1.0 / 32, "jitter": 1, "skip_rule": 1}
+    # in/ex 1 everywhere: every ray that reaches the volume is excluded
+    rgba, f, depth = orc.cast2d_inex(vol, spacing, _uniform_2d_tf(N, 0.5), np.ones((N, N), np.float32), True, vd,
+                                     march, "f64")
+    assert not rgba.any() and not f.any()
+    # ... and ends one step in (depth j + 1 for a ray that starts inside)
+    jt = orc.jitter_table()
+    assert depth[10, 12] == pytest.approx(1.0 + float(jt[12 + 16 * 10]), abs=1e-12)
+    # in/ex 0.3, opacity 0: nothing composited; black keyhole -> opaque black
+    tf0 = _uniform_2d_tf(N, 0.0)
+    ie = np.full((N, N), 0.3, np.float32)
+    wide = configs.view_dict(_parallel_z_view(dims, spacing, 24, 20)[0])  # corners miss the volume
+    r_plain, _, _ = orc.cast2d_inex(vol, spacing, tf0, ie, False, wide, march, "f64")
+    r_black, _, _ = orc.cast2d_inex(vol, spacing, tf0, ie, True, wide, march, "f64")
+    assert not r_plain.any()
+    hit = r_black[..., 3] > 0
+    assert hit[8:12, 10:14].all() and not r_black[..., :3].any()
+    assert np.all(r_black[..., 3][hit] == 255)
+    assert not hit[0, 0] and not hit[-1, -1]
+
+
+def test_inex_keyhole_range_admits_inside_rays():
+    """With a keyhole box the exclusion window [ee, es] holds the samples
+    inside it: a ray whose first sample lies in the box is accumulated in full
+    (later samples are never tested again), a ray outside it is excluded.
+    With in/ex = 1 and uniform opacity, pixels either match the plain 2D
+    caster or are empty, and both kinds occur."""
+    dims, spacing = (16, 18, 20), (1.0, 1.0, 1.0)
+    view = _parallel_z_view(dims, spacing, 24, 20, scale=0.25 * dims[1])[0]
+    # keyhole box: about x in [5, 11], y in [6, 12], z above 10 (the camera
+    # looks down -z), planes a x + b y + c z + d >= 0 inside.  The side planes
+    # are tilted a little: a plane parallel to a ray never excludes it (its
+    # crossing parameter divides by zero), as in the reference.
+    key = [(1, 0, 0.01, -5.0), (-1, 0, 0.01, 11.0), (0, 1, 0.01, -6.0), (0, -1, 0.01, 12.0), (0, 0, 1, -10.0),
+           (0, 0, -1, 40.0)]
+    view.n_keyhole_planes = 6
+    view.keyhole_planes[:] = [float(x) for p in key for x in p]
+    vd = configs.view_dict(view)
+    vol = np.full(dims[::-1], 1000, np.uint16)
+    N = 32
+    tf = _uniform_2d_tf(N, 0.1)
+    march = {"step_scale": 0.5, "term_T": 1.0 / 32, "jitter": 0, "skip_rule": 1}
+    ie_all = np.ones((N, N), np.float32)
+    r_in, f_in, _ = orc.cast2d_inex(vol, spacing, tf, ie_all, False, vd, march, "f64")
+    r_ref, f_ref, _ = orc.cast2d_inex(vol, spacing, tf, np.zeros((N, N), np.float32), False, vd, march, "f64")
+    rays = orc.form_rays(dims, spacing, vd, march, "f64")
+    k = np.floor(rays["num_steps"]).astype(np.int64)
+    es = k - np.ceil(rays["exclude_start"]).astype(np.int64)
+    ee = k - np.floor(rays["exclude_end"]).astype(np.int64)
+    # the first accumulated sample is taken with k - 1 steps left
+    admitted = (es >= k - 1) & (ee <= k - 1) & (k > 0)
+    full = np.all(f_in == f_ref, axis=-1)
+    empty = ~f_in.any(axis=-1)
+    assert np.all(full[admitted]) and np.all(empty[~admitted & (k > 0)])
+    assert admitted.sum() >= 10 and (~admitted & (k > 0)).sum() >= 10
+
+
+def test_dual_uniform_closed_form_and_axes():
+    """Dual-image caster: component 1 picks the table column, component 2 the
+    row (an opacity entry at the transposed position renders nothing); the
+    step count is trunc(floor(numSteps) - j), so a jittered ray takes one
+    sample fewer than an unjittered one."""
+    dims, spacing = (16, 18, 20), (1.0, 1.0, 1.0)
+    view, _, _ = _parallel_z_view(dims, spacing, 24, 20, scale=0.25 * dims[1])
+    vd = configs.view_dict(view)
+    v1 = np.full(dims[::-1], 1000, np.uint16)
+    v2 = np.full(dims[::-1], 3000, np.uint16)
+    N = 32
+    params = {"intensity_low": 0.0, "intensity_multiplier": float(np.float32(1 / 4095.0)), "gradient_low": 0.0,
+              "gradient_multiplier": float(np.float32(1 / 4095.0)), "gradient_offset": 1.0}
+    c1, c2 = int(1000 / 4095.0 * N), int(3000 / 4095.0 * N)
+    assert c1 != c2
+    a = 0.05
+    tables = {k: np.zeros((N, N), np.float32) for k in orc.TF2D_NAMES}
+    tables["alpha"][c2, c1] = a
+    tables["green"][:] = 0.8
+    tables["ambient"][:] = 1.0
+    tables["specular_power"][:] = 1.0
+    tf = {"tables": tables, "params": params}
+    m = math.floor((dims[2] - 3) / 0.5)
+    g32 = float(np.float32(0.8))
+    a32 = float(np.float32(a))
+    jt = orc.jitter_table()
+    for jit in (0, 1):
+        march = {"step_scale": 0.5, "term_T": 0.0, "jitter": jit, "skip_rule": 1}
+        _, f, depth = orc.cast_dual(v1, v2, spacing, tf, vd, march, "f64")
+        for y in range(5, 15):
+            for x in range(7, 17):
+                j = float(np.float32(jt[(x % 16) + 16 * (y % 16)])) if jit else 0.0
+                n = m if j == 0.0 else m - 1
+                T = (1 - a32) ** n
+                assert depth[y, x] == n
+                assert f[y, x, 1] == pytest.approx(g32 * (1 - T), rel=1e-12)
+                assert f[y, x, 3] == pytest.approx(1 - T, rel=1e-12)
+    # transposed table: nothing
+    tables2 = dict(tables, alpha=np.ascontiguousarray(tables["alpha"].T))
+    rgba, _, _ = orc.cast_dual(v1, v2, spacing, {"tables": tables2, "params": params}, vd,
+                               {"step_scale": 0.5, "term_T": 0.0, "jitter": 0, "skip_rule": 1}, "f64")
+    assert not rgba.any()
+    # swapped volumes: nothing either
+    rgba, _, _ = orc.cast_dual(v2, v1, spacing, tf, vd,
+                               {"step_scale": 0.5, "term_T": 0.0, "jitter": 0, "skip_rule": 1}, "f64")
+    assert not rgba.any()
+
+
+def test_dual_reduces_to_2d_caster():
+    """With component 2 equal to a constant and its mapping sending it to
+    row 0, the dual caster samples the 2D table exactly where the 2D caster
+    with an all-gradients-to-row-0 mapping does; without jitter the step counts
+    agree, so the images are identical (float64 and float32)."""
+    s = configs.scene_c3(scale=0.06)
+    vd = configs.view_dict(abi.compute_view(s.extra["camera"], 40, 30, spacing=s.spacing))
+    N = s.tf2d["tables"]["alpha"].shape[0]
+    tables = {k: np.ascontiguousarray(np.repeat(s.tf2d["tables"][k][:1], N, axis=0)) for k in orc.TF2D_NAMES}
+    p = dict(s.tf2d["params"])
+    tf2 = {"tables": tables, "params": p}
+    v2 = np.zeros_like(s.volume)
+    dual_p = dict(p, gradient_low=0.0, gradient_multiplier=0.0)
+    march = dict(s.march, jitter=0)
+    for prec in ("f64", "f32"):
+        a = orc.cast2d(s.volume, s.spacing, tf2, vd, march, prec)
+        b = orc.cast_dual(s.volume, v2, s.spacing, {"tables": tables, "params": dual_p}, vd, march, prec)
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_post_shade_inex_alpha_fill():
+    """In/Ex cel shade: squared depth differences (no root); a pixel with
+    zero alpha gets alpha (signed char)(255 - 255 g): 0 where flat, and the
+    signed-char clamp 127 where the edge term blacks it out fully."""
+    W, H = 6, 4
+    depth = np.full((H, W), 5.0, np.float32)
+    depth[:, 3:] = 7.0
+    mn = np.zeros((H, W), np.float32)
+    mx = np.full((H, W), 10.0, np.float32)
+    rgba = np.zeros((H, W, 4), np.uint8)
+    rgba[:, :, 0] = 200
+    rgba[0, :, 3] = 255
+    shade = {"celr": 1.0, "cela": 0.0, "celc": 100.0, "disr": 0.0, "disa": 0.0, "disc": 0.0}
+    d, out = orc.post_shade_inex(shade, depth, mn, mx, rgba)
+    np.testing.assert_allclose(d, depth / 10.0)
+    # column 2 sees the 0.5 -> 0.7 step: g = 1 - sat(0.04 * 100) = 0
+    assert np.all(out[:, 2, 0] == 0) and np.all(out[1:, 2, 3] == 127)
+    assert np.all(out[0, :, 3] == 255)  # non-zero alpha is kept
+    # flat columns: g = 1, colour kept, alpha 255 - 255 = 0
+    assert np.all(out[:, 0, 0] == 200) and np.all(out[1:, 0, 3] == 0)
+    # a half edge: celc 12.5 -> g = 1 - 0.5, alpha (char)127.5 -> 127, colour 100
+    d, out = orc.post_shade_inex(dict(shade, celc=12.5), depth, mn, mx, rgba)
+    assert np.all(out[:, 2, 0] == 100) and np.all(out[1:, 2, 3] == 127)
+    # a quarter: celc 6.25 -> g = 0.75, alpha trunc(63.75) = 63
+    d, out = orc.post_shade_inex(dict(shade, celc=6.25), depth, mn, mx, rgba)
+    assert np.all(out[:, 2, 0] == 150) and np.all(out[1:, 2, 3] == 63)
+
+
 def test_flip_symmetry():
     s = configs.scene_c1(scale=0.25)
     W, H = 48, 40

@@ -117,6 +117,14 @@ typedef struct rvk_tf2d_params {
   float gradient_low, gradient_multiplier, gradient_offset;
 } rvk_tf2d_params;
 
+/* Dual-image TF scalars: each component's low end and 1/range
+ * (V/CUDA_containerDualImageTransferFunctionInformation.h:20-30); component 1
+ * picks the table column, component 2 the row. */
+typedef struct rvk_dual_params {
+  float intensity1_low, intensity1_multiplier;
+  float intensity2_low, intensity2_multiplier;
+} rvk_dual_params;
+
 /* Optional per-pixel side outputs of the compositing casters (device
  * pointers, each W*H floats, any may be NULL): steps taken, and the min/max
  * ray depths formRays writes (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:373,387). */
@@ -200,6 +208,12 @@ int rvk_tf1d_destroy(rvk_tf1d* tf);
 int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const float* const keyhole_tables[8],
                     const rvk_tf2d_params* params, rvk_tf2d** out);
 int rvk_tf2d_destroy(rvk_tf2d* tf);
+/* In/Ex logic table for rvk_render_2d_inex: n*n floats indexed like the
+ * other tables (the reference loads it beside them,
+ * V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.h:35-45).  The caster only
+ * tests inEx > 0 and inEx > 0.5, so the table is kept as those two bits per
+ * cell.  NULL removes it. */
+int rvk_tf2d_set_inex(rvk_tf2d* tf, const float* inex_table);
 
 /* ---- renders ----------------------------------------------------------------
  * Each fuses formRays into the caster (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:391-447).
@@ -213,6 +227,20 @@ int rvk_render_1d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf1d
  * then takes the Phong-Lambert factor as 1). */
 int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d* tf, const rvk_view* view,
                   const rvk_march_params* march, int use_gradmag, uint8_t* out_rgba, const rvk_aux_out* aux);
+/* Replaces CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo_doRender
+ * (V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.h:29-33): the 2D caster
+ * with in/ex logic (tf needs rvk_tf2d_set_inex; its keyhole tables, if any,
+ * are not used).  black_keyhole: rays that composited anything are opaque. */
+int rvk_render_2d_inex(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d* tf, const rvk_view* view,
+                       const rvk_march_params* march, int black_keyhole, uint8_t* out_rgba, const rvk_aux_out* aux);
+/* Replaces CUDA_vtkCudaDualImageVolumeMapper_renderAlgo_doRender
+ * (V/CUDA_vtkCudaDualImageVolumeMapper_renderAlgo.h:29-33): the two components
+ * of the reference's 2-component image as two volumes of one size, type and
+ * cell layout (frame applies to both).  tf's own params are not used; its
+ * keyhole tables select the keyhole-TF variant. */
+int rvk_render_dual(rvk_ctx* ctx, const rvk_volume* vol1, const rvk_volume* vol2, int frame, const rvk_tf2d* tf,
+                    const rvk_dual_params* params, const rvk_view* view, const rvk_march_params* march,
+                    uint8_t* out_rgba, const rvk_aux_out* aux);
 /* Replaces CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo_doRender
  * (V/CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo.h:30-35).  out is W*H
  * RGBA8 or W*H floats according to drr->output. */
@@ -223,6 +251,11 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
  * past the last row/column are clamped (the reference reads past them). */
 int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
                    const float* min_depth, const float* max_depth, uint8_t* rgba);
+/* The In/Ex-logic mapper's cel shade (V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.cuh:182-239):
+ * squared depth differences, and pixels with zero alpha get
+ * alpha = (signed char)(255 - 255 g) as a byte. */
+int rvk_post_shade_inex(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
+                        const float* min_depth, const float* max_depth, uint8_t* rgba);
 
 /* ---- batched DRR and its adjoint (no counterpart in the reference) ----------
  * A plan holds many views of one volume frame -- rigid poses and detectors of

@@ -57,6 +57,11 @@ class Tf2dParams(C.Structure):
                 ("gradient_offset", C.c_float)]
 
 
+class DualParams(C.Structure):
+    _fields_ = [("intensity1_low", C.c_float), ("intensity1_multiplier", C.c_float),
+                ("intensity2_low", C.c_float), ("intensity2_multiplier", C.c_float)]
+
+
 class AuxOut(C.Structure):
     _fields_ = [("depth", C.c_void_p), ("min_depth", C.c_void_p), ("max_depth", C.c_void_p)]
 
@@ -102,10 +107,16 @@ _SIGS = {
     "rvk_tf1d_destroy": (_I, [_P]),
     "rvk_tf2d_create": (_I, [_P, _I, C.POINTER(_P), C.POINTER(_P), C.POINTER(Tf2dParams), C.POINTER(_P)]),
     "rvk_tf2d_destroy": (_I, [_P]),
+    "rvk_tf2d_set_inex": (_I, [_P, _P]),
     "rvk_render_1d": (_I, [_P, _P, _I, _P, C.POINTER(View), C.POINTER(MarchParams), _P, C.POINTER(AuxOut)]),
     "rvk_render_2d": (_I, [_P, _P, _I, _P, C.POINTER(View), C.POINTER(MarchParams), _I, _P, C.POINTER(AuxOut)]),
     "rvk_render_drr": (_I, [_P, _P, _I, C.POINTER(View), C.POINTER(MarchParams), C.POINTER(DrrParams), _P]),
+    "rvk_render_2d_inex": (_I, [_P, _P, _I, _P, C.POINTER(View), C.POINTER(MarchParams), _I, _P,
+                                C.POINTER(AuxOut)]),
+    "rvk_render_dual": (_I, [_P, _P, _P, _I, _P, C.POINTER(DualParams), C.POINTER(View), C.POINTER(MarchParams), _P,
+                             C.POINTER(AuxOut)]),
     "rvk_post_shade": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
+    "rvk_post_shade_inex": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
     "rvk_count_ray_steps": (_I, [_P, _P, C.POINTER(View), C.POINTER(MarchParams), C.POINTER(C.c_int64)]),
     "rvk_drr_plan_create": (_I, [_P, _P, _I, _I, C.POINTER(View), _P, C.POINTER(MarchParams), C.POINTER(DrrParams),
                                  C.POINTER(_P)]),
@@ -328,6 +339,17 @@ class TF2D:
         self.handle = h
         self.n = n
 
+    def set_inex(self, table) -> None:
+        """In/Ex logic table ((n, n) floats indexed [g, i]) for render_2d_inex;
+        None removes it."""
+        if table is None:
+            check(lib().rvk_tf2d_set_inex(self.handle, None))
+            return
+        t = np.ascontiguousarray(table, dtype=np.float32)
+        if t.shape != (self.n, self.n):
+            raise ValueError(f"in/ex table must be ({self.n}, {self.n}), got {t.shape}")
+        check(lib().rvk_tf2d_set_inex(self.handle, t.ctypes.data))
+
     def close(self):
         if self.handle:
             lib().rvk_tf2d_destroy(self.handle)
@@ -355,6 +377,20 @@ def render_2d(ctx: Context, vol: Volume, tf: TF2D, view: View, march: MarchParam
               aux: AuxOut | None = None, frame: int = 0):
     check(lib().rvk_render_2d(ctx.handle, vol.handle, frame, tf.handle, C.byref(view), C.byref(march),
                               int(use_gradmag), C.c_void_p(dptr(out)), C.byref(aux) if aux is not None else None))
+
+
+def render_2d_inex(ctx: Context, vol: Volume, tf: TF2D, view: View, march: MarchParams, out,
+                   black_keyhole: bool = False, aux: AuxOut | None = None, frame: int = 0):
+    check(lib().rvk_render_2d_inex(ctx.handle, vol.handle, frame, tf.handle, C.byref(view), C.byref(march),
+                                   int(black_keyhole), C.c_void_p(dptr(out)),
+                                   C.byref(aux) if aux is not None else None))
+
+
+def render_dual(ctx: Context, vol1: Volume, vol2: Volume, tf: TF2D, params: DualParams, view: View,
+                march: MarchParams, out, aux: AuxOut | None = None, frame: int = 0):
+    check(lib().rvk_render_dual(ctx.handle, vol1.handle, vol2.handle, frame, tf.handle, C.byref(params),
+                                C.byref(view), C.byref(march), C.c_void_p(dptr(out)),
+                                C.byref(aux) if aux is not None else None))
 
 
 def count_ray_steps(ctx: Context, vol: Volume, view: View, march: MarchParams) -> int:

@@ -331,7 +331,12 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
 // ===========================================================================
 // Post passes (V/...renderAlgo.cu:449-512), fused into one kernel per pixel.
 // The depth normalisation of a pixel's neighbours is recomputed locally, so
-// no pass ordering between pixels is needed.
+// no pass ordering between pixels is needed.  INEX is the In/Ex-logic
+// mapper's cel shade (V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.cuh:182-239):
+// the squared depth difference without the square root, and pixels with zero
+// alpha get alpha = (char)(255 - 255 g), a float-to-signed-char conversion
+// (truncating, clamped to [-128, 127], NaN -> 0) stored as a byte.
+template <bool INEX>
 __global__ __launch_bounds__(256) void post_shade_kernel(int W, int H, rvk_shade_params s, const float* depth,
                                                          const float* mind, const float* maxd, uint32_t* rgba,
                                                          float* depth_out) {
@@ -346,13 +351,20 @@ __global__ __launch_bounds__(256) void post_shade_kernel(int W, int H, rvk_shade
   const float d0 = norm(i);
   const float dy = norm((long)min(y + 1, H - 1) * W + x);
   const float dx = norm((long)y * W + min(x + 1, W - 1));
-  float gm = sqrt_rz((dx - d0) * (dx - d0) + (dy - d0) * (dy - d0));
+  float gm = (dx - d0) * (dx - d0) + (dy - d0) * (dy - d0);
+  if (!INEX) gm = sqrt_rz(gm);
   gm = 1.0f - s.celr * satf((gm - s.cela) * s.celc);
   gm *= 1.0f - s.disr * satf((d0 - s.disa) * s.disc);
   const uint32_t c = rgba[i];
   const uint32_t R = to_u8(gm * (float)(c & 255u)), G = to_u8(gm * (float)((c >> 8) & 255u));
   const uint32_t B = to_u8(gm * (float)((c >> 16) & 255u));
-  rgba[i] = R | (G << 8) | (B << 16) | (c & 0xff000000u);
+  uint32_t A = c & 0xff000000u;
+  if (INEX && A == 0u) {
+    const float v = 255.0f - 255.0f * gm;
+    const int t = (v == v) ? (int)fmaxf(-128.0f, fminf(127.0f, truncf(v))) : 0;
+    A = (uint32_t)(uint8_t)(int8_t)t << 24;
+  }
+  rgba[i] = R | (G << 8) | (B << 16) | A;
   if (depth_out) depth_out[i] = d0;
 }
 
@@ -375,21 +387,6 @@ __global__ __launch_bounds__(256) void count_steps_kernel(VolGeom g, ViewGeom v,
 }
 
 // ===========================================================================
-static int check_march(const rvk_march_params* m) {
-  if (!m) return fail(RVK_ERR_INVALID_ARGUMENT, "march params are NULL");
-  if (!(m->step_scale > 0.0f) || !std::isfinite(m->step_scale))
-    return fail(RVK_ERR_INVALID_ARGUMENT, "step_scale must be positive and finite");
-  return RVK_OK;
-}
-
-// Workgroups for a view: whole super-tiles of 16x16-pixel tiles; `stiles_x`
-// receives the number of super-tile columns.
-static int launch_grid(const rvk_view* v, int& stiles_x) {
-  const int e = 16 << kSuperLog2;
-  stiles_x = (v->width + e - 1) / e;
-  return stiles_x * ((v->height + e - 1) / e) << (2 * kSuperLog2);
-}
-
 }  // namespace rvk
 
 using namespace rvk;
@@ -534,8 +531,8 @@ int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d
   });
 }
 
-int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
-                   const float* min_depth, const float* max_depth, uint8_t* rgba) {
+static int post_shade(rvk_ctx* ctx, bool inex, int width, int height, const rvk_shade_params* shade, float* depth,
+                      const float* min_depth, const float* max_depth, uint8_t* rgba) {
   if (!ctx || !shade || !depth || !min_depth || !max_depth || !rgba)
     return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (width <= 0 || height <= 0) return fail(RVK_ERR_INVALID_ARGUMENT, "bad image size");
@@ -545,12 +542,26 @@ int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* 
   // neighbours, so the normalised copy goes to a second buffer first.
   float* tmp = nullptr;
   RVK_HIP(hipMallocAsync((void**)&tmp, sizeof(float) * (size_t)width * height, ctx->stream));
-  hipLaunchKernelGGL(post_shade_kernel, grid, dim3(256), 0, ctx->stream, width, height, *shade, depth, min_depth,
-                     max_depth, (uint32_t*)rgba, tmp);
+  if (inex)
+    hipLaunchKernelGGL(post_shade_kernel<true>, grid, dim3(256), 0, ctx->stream, width, height, *shade, depth,
+                       min_depth, max_depth, (uint32_t*)rgba, tmp);
+  else
+    hipLaunchKernelGGL(post_shade_kernel<false>, grid, dim3(256), 0, ctx->stream, width, height, *shade, depth,
+                       min_depth, max_depth, (uint32_t*)rgba, tmp);
   RVK_HIP(hipGetLastError());
   RVK_HIP(hipMemcpyAsync(depth, tmp, sizeof(float) * (size_t)width * height, hipMemcpyDeviceToDevice, ctx->stream));
   RVK_HIP(hipFreeAsync(tmp, ctx->stream));
   return RVK_OK;
+}
+
+int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
+                   const float* min_depth, const float* max_depth, uint8_t* rgba) {
+  return post_shade(ctx, false, width, height, shade, depth, min_depth, max_depth, rgba);
+}
+
+int rvk_post_shade_inex(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
+                        const float* min_depth, const float* max_depth, uint8_t* rgba) {
+  return post_shade(ctx, true, width, height, shade, depth, min_depth, max_depth, rgba);
 }
 
 }  // extern "C"

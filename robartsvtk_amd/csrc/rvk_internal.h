@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "rvk.h"
 #include "rvk_device.h"
@@ -54,6 +55,13 @@ struct rvk_tf2d {
   uint32_t* kocc = nullptr;
   int occ_words = 0;
   rvk_tf2d_params p{};
+  // In/Ex logic (rvk_tf2d_set_inex).  The caster only compares the table
+  // with 0 and 0.5, so it keeps per cell bit 0 = (inEx > 0 || alpha > 0)
+  // and bit 1 = (inEx > 0.5): as bytes, and for n <= kTf2dOccMaxN also as
+  // two bitmaps (occ_words each) staged in LDS.
+  uint8_t* inex_flags = nullptr;
+  uint32_t* inex_occ = nullptr;
+  std::vector<float> alpha_host;  // the opacity table, kept for the flags
 };
 
 // Largest 2D TF whose opacity bitmap the caster stages in LDS (8 KiB; 16 KiB
@@ -96,5 +104,20 @@ float min_spacing(const rvk_volume* v);
 int check_view(const rvk_view* v);
 int check_frame(const rvk_volume* v, int frame);
 int ensure_scratch(rvk_ctx* ctx, size_t bytes);
+
+inline int check_march(const rvk_march_params* m) {
+  if (!m) return fail(RVK_ERR_INVALID_ARGUMENT, "march params are NULL");
+  if (!(m->step_scale > 0.0f) || !(m->step_scale < __builtin_huge_valf()))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "step_scale must be positive and finite");
+  return RVK_OK;
+}
+
+// Workgroups for a caster launch over a view: whole super-tiles of
+// 16x16-pixel tiles; `stiles_x` receives the number of super-tile columns.
+inline int launch_grid(const rvk_view* v, int& stiles_x) {
+  const int e = 16 << kSuperLog2;
+  stiles_x = (v->width + e - 1) / e;
+  return stiles_x * ((v->height + e - 1) / e) << (2 * kSuperLog2);
+}
 
 }  // namespace rvk

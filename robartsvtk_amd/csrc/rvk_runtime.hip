@@ -565,10 +565,51 @@ int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const flo
     if (t->cells) hipFree(t->cells);
     if (t->kcells) hipFree(t->kcells);
     if (t->occ) hipFree(t->occ);
+    if (t->kocc) hipFree(t->kocc);
     delete t;
     return rc;
   }
+  t->alpha_host.assign(tables[0], tables[0] + (size_t)n * n);
   *out = t;
+  return RVK_OK;
+}
+
+// The In/Ex table reduced to the two comparisons the caster makes
+// (V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.cuh:111,125-126):
+// bit 0 = (inEx > 0 || alpha > 0), bit 1 = (inEx > 0.5).
+int rvk_tf2d_set_inex(rvk_tf2d* t, const float* inex) {
+  if (!t) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  RVK_HIP(hipSetDevice(t->ctx->device));
+  RVK_HIP(hipStreamSynchronize(t->ctx->stream));
+  if (t->inex_flags) { hipFree(t->inex_flags); t->inex_flags = nullptr; }
+  if (t->inex_occ) { hipFree(t->inex_occ); t->inex_occ = nullptr; }
+  if (!inex) return RVK_OK;
+  const size_t nn = (size_t)t->n * t->n;
+  std::vector<uint8_t> flags(nn);
+  for (size_t c = 0; c < nn; ++c)
+    flags[c] = (uint8_t)(((inex[c] > 0.0f || t->alpha_host[c] > 0.0f) ? 1u : 0u) | (inex[c] > 0.5f ? 2u : 0u));
+  RVK_HIP(hipMalloc((void**)&t->inex_flags, nn));
+  hipError_t e = hipMemcpy(t->inex_flags, flags.data(), nn, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(t->inex_flags);
+    t->inex_flags = nullptr;
+    return hip_fail(e, "hipMemcpy(inex flags)");
+  }
+  if (t->occ_words) {
+    const size_t w = (size_t)t->occ_words;
+    std::vector<uint32_t> bits(2 * w, 0u);
+    for (size_t c = 0; c < nn; ++c) {
+      if (flags[c] & 1u) bits[c >> 5] |= 1u << (c & 31);
+      if (flags[c] & 2u) bits[w + (c >> 5)] |= 1u << (c & 31);
+    }
+    RVK_HIP(hipMalloc((void**)&t->inex_occ, 2 * w * sizeof(uint32_t)));
+    e = hipMemcpy(t->inex_occ, bits.data(), 2 * w * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(t->inex_occ);
+      t->inex_occ = nullptr;
+      return hip_fail(e, "hipMemcpy(inex bitmaps)");
+    }
+  }
   return RVK_OK;
 }
 
@@ -580,6 +621,8 @@ int rvk_tf2d_destroy(rvk_tf2d* t) {
   if (t->kcells) hipFree(t->kcells);
   if (t->occ) hipFree(t->occ);
   if (t->kocc) hipFree(t->kocc);
+  if (t->inex_flags) hipFree(t->inex_flags);
+  if (t->inex_occ) hipFree(t->inex_occ);
   delete t;
   return RVK_OK;
 }

@@ -62,6 +62,52 @@ def gpu_2d(ctx, vol_h, tf: dict, view: abi.View, march: dict, use_gradmag: bool,
     return out.cpu().numpy()
 
 
+def _aux(with_aux, H, W):
+    import torch
+    if not with_aux:
+        return None, None
+    bufs = [torch.zeros((H, W), dtype=torch.float32, device="cuda") for _ in range(3)]
+    return abi.AuxOut(*[b.data_ptr() for b in bufs]), bufs
+
+
+def gpu_2d_inex(ctx, vol_h, tf: dict, inex: np.ndarray, black_keyhole: bool, view: abi.View, march: dict,
+                with_aux: bool = False):
+    import torch
+    W, H = view.width, view.height
+    tfh = abi.TF2D(ctx, tf["tables"], configs.tf2d_params(tf["params"]))
+    tfh.set_inex(inex)
+    out = torch.full((H, W, 4), 77, dtype=torch.uint8, device="cuda")
+    aux, bufs = _aux(with_aux, H, W)
+    abi.render_2d_inex(ctx, vol_h, tfh, view, configs.march_params(march), out, black_keyhole, aux)
+    torch.cuda.synchronize()
+    tfh.close()
+    if with_aux:
+        return out.cpu().numpy(), [b.cpu().numpy() for b in bufs]
+    return out.cpu().numpy()
+
+
+def dual_params(p: dict) -> abi.DualParams:
+    """Component 1 from the intensity fields, component 2 from the gradient
+    fields of an oracle-style params dict (the oracle's dual caster reads them
+    so)."""
+    return abi.DualParams(p["intensity_low"], p["intensity_multiplier"], p["gradient_low"],
+                          p["gradient_multiplier"])
+
+
+def gpu_dual(ctx, v1_h, v2_h, tf: dict, view: abi.View, march: dict, with_aux: bool = False):
+    import torch
+    W, H = view.width, view.height
+    tfh = abi.TF2D(ctx, tf["tables"], configs.tf2d_params(tf["params"]), tf.get("keyhole_tables"))
+    out = torch.full((H, W, 4), 77, dtype=torch.uint8, device="cuda")
+    aux, bufs = _aux(with_aux, H, W)
+    abi.render_dual(ctx, v1_h, v2_h, tfh, dual_params(tf["params"]), view, configs.march_params(march), out, aux)
+    torch.cuda.synchronize()
+    tfh.close()
+    if with_aux:
+        return out.cpu().numpy(), [b.cpu().numpy() for b in bufs]
+    return out.cpu().numpy()
+
+
 # ---------------------------------------------------------------------------
 # comparisons
 def rel_err(a: np.ndarray, b: np.ndarray) -> np.ndarray:

@@ -1,0 +1,214 @@
+"""Parity of the sibling compositing casters -- the 2D caster with in/ex
+logic and the dual-image caster -- and the In/Ex cel shade (C-ABI on the
+GPU) with the CPU oracle.
+
+Tolerances as for the other compositing casters: RGBA8 within 1 LSB on
+>= 99.9 % of pixels against the FP32 oracle and against the FP64 oracle
+replaying the FP32 branch decisions; the post shade bit-exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from robartsvtk_amd import abi, configs
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_rgba(g, o32, o64, label):
+    f32, m32 = H.lsb_stats(g, o32)
+    f64, m64 = H.lsb_stats(g, o64)
+    exact = float(np.mean(np.all(g == o32, axis=-1)))
+    print(f"{label}: within 1 LSB vs f32 {f32:.6f} (max {m32}, bit-exact {exact:.6f}); vs f64 {f64:.6f} (max {m64})")
+    assert f32 >= 0.999 and f64 >= 0.999
+
+
+def _scene(W=240, H=135):
+    s = configs.scene_c3(scale=0.25)
+    view = abi.compute_view(s.extra["camera"], W, H, spacing=s.spacing)
+    return s, view
+
+
+def _keyhole(view, n):
+    key = [(1, 0, 0, -0.3 * n), (-1, 0, 0, 0.7 * n), (0, 1, 0, -0.3 * n), (0, -1, 0, 0.7 * n),
+           (0, 0, 1, -0.2 * n), (0, 0, -1, 0.8 * n)]
+    view.n_keyhole_planes = 6
+    view.keyhole_planes[:] = [float(x) for p in key for x in p]
+
+
+def _inex_tf(tables, params):
+    """An In/Ex setup with every outcome on the c3 scene: intensities from
+    0.2 of the range up are opaque-ish and excluding (in/ex 1), 0.1-0.2 marks
+    rays without opacity (in/ex 0.3, alpha 0), below is empty.  A ray is then
+    excluded if it meets the upper range first outside the keyhole window,
+    composited if it meets it inside, and marked (black with the black
+    keyhole) if it crosses the marking band first."""
+    N = tables["alpha"].shape[0]
+    c, b = int(0.2 * N), int(0.1 * N)
+    alpha = np.array(tables["alpha"], np.float32)
+    alpha[:, :c] = 0.0
+    inex = np.zeros((N, N), np.float32)
+    inex[:, c:] = 1.0
+    inex[:, b:c] = 0.3
+    return {"tables": dict(tables, alpha=alpha), "params": params}, inex
+
+
+# ---------------------------------------------------------------------------
+# In/Ex logic
+@pytest.mark.parametrize("skip,black,key,precise", [(1, False, True, 0), (0, True, True, 0), (1, True, False, 1),
+                                                    (1, True, True, 1)])
+def test_inex_parity(gpu_ctx, skip, black, key, precise):
+    s, view = _scene()
+    if key:
+        _keyhole(view, s.dims[0])
+    tf, inex = _inex_tf(s.tf2d["tables"], s.tf2d["params"])
+    march = dict(s.march, skip_rule=skip, term_T=1.0 / 32, precise=precise)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g, (depth, _, _) = H.gpu_2d_inex(gpu_ctx, vh, tf, inex, black, view, march, with_aux=True)
+    vd = configs.view_dict(view)
+    o32, _, od = orc.cast2d_inex(s.volume, s.spacing, tf, inex, black, vd, march, "f32")
+    o64, _, _ = orc.cast2d_inex(s.volume, s.spacing, tf, inex, black, vd, dict(march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, f"inex skip={skip} black={black} key={key} precise={precise}")
+    assert np.mean(depth == od) >= 0.999
+    # every outcome occurs: rays excluded (empty although the plain 2D caster
+    # composites there), composited, and marked black
+    plain, _, _ = orc.cast2d(s.volume, s.spacing, tf, vd, march, "f32")
+    excluded = (o32[..., 3] == 0) & (plain[..., 3] > 0)
+    admitted = (o32[..., 3] > 0) & (plain[..., 3] > 0)
+    assert excluded.sum() > 1000 and admitted.sum() > 1000
+    if black:
+        assert ((o32[..., 3] == 255) & (plain[..., 3] == 0)).sum() > 50
+    vh.close()
+
+
+def test_inex_large_table_global_flags(gpu_ctx):
+    """A 300x300 table is past the LDS bitmaps: the flags come from global
+    memory; same answers."""
+    s, view = _scene(120, 68)
+    _keyhole(view, s.dims[0])
+    tf, inex = _inex_tf(configs.tf2d_bumps(17, 300), s.tf2d["params"])
+    march = dict(s.march, term_T=1.0 / 32)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g = H.gpu_2d_inex(gpu_ctx, vh, tf, inex, True, view, march)
+    vd = configs.view_dict(view)
+    o32, _, _ = orc.cast2d_inex(s.volume, s.spacing, tf, inex, True, vd, march, "f32")
+    o64, _, _ = orc.cast2d_inex(s.volume, s.spacing, tf, inex, True, vd, dict(march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, "inex n=300")
+    vh.close()
+
+
+def test_inex_post_shade(gpu_ctx):
+    import torch
+    s, view = _scene()
+    _keyhole(view, s.dims[0])
+    tf, inex = _inex_tf(s.tf2d["tables"], s.tf2d["params"])
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    march = dict(s.march, term_T=1.0 / 32)
+    g, (depth, mind, maxd) = H.gpu_2d_inex(gpu_ctx, vh, tf, inex, False, view, march, with_aux=True)
+    vd = configs.view_dict(view)
+    rays = orc.form_rays(s.dims, s.spacing, vd, march, "f32")
+    assert np.array_equal(mind, rays["min_depth"]) and np.array_equal(maxd, rays["max_depth"])
+    shade = {"celr": 0.7, "cela": 1e-5, "celc": 1.0 / (4e-4 - 1e-5), "disr": 0.25, "disa": 0.45,
+             "disc": 1.0 / (0.6 - 0.45)}
+    rgba = torch.from_numpy(np.ascontiguousarray(g)).cuda()
+    dd, dmin, dmax = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (depth, mind, maxd))
+    sp = abi.ShadeParams(*[shade[k] for k in ("celr", "cela", "celc", "disr", "disa", "disc")])
+    abi.check(abi.lib().rvk_post_shade_inex(gpu_ctx.handle, g.shape[1], g.shape[0], sp, dd.data_ptr(),
+                                            dmin.data_ptr(), dmax.data_ptr(), rgba.data_ptr()))
+    torch.cuda.synchronize()
+    od, orgba = orc.post_shade_inex(shade, depth, mind, maxd, g)
+    out = rgba.cpu().numpy()
+    assert np.array_equal(out, orgba)
+    assert np.array_equal(dd.cpu().numpy(), od)
+    # the alpha fill touched empty pixels on depth edges
+    assert ((g[..., 3] == 0) & (out[..., 3] != 0)).sum() > 0
+    vh.close()
+
+
+# ---------------------------------------------------------------------------
+# dual image
+def _dual_scene():
+    s, view = _scene()
+    v2 = abi.synth_blobs_perlin_u16(s.dims, 11, n_blobs=4, base_period=20.0, octaves=3, vmax=4095.0)
+    N = 128
+    f32 = lambda x: float(np.float32(x))
+    params = {"intensity_low": 0.0, "intensity_multiplier": f32(1.0 / 4095.0), "gradient_low": 0.0,
+              "gradient_multiplier": f32(1.0 / 4095.0), "gradient_offset": 1.0}
+    return s, view, v2, {"tables": _thin(configs.tf2d_bumps(23, N)), "params": params}
+
+
+def _thin(tables):
+    """Opacity 0 below 0.1, scaled by 0.05 above: rays reach the keyhole."""
+    a = tables["alpha"]
+    return dict(tables, alpha=np.where(a > 0.1, a * 0.05, 0.0).astype(np.float32))
+
+
+@pytest.mark.parametrize("skip,keytf,precise", [(1, False, 0), (0, False, 0), (1, True, 0), (1, True, 1)])
+def test_dual_parity(gpu_ctx, skip, keytf, precise):
+    s, view, v2, tf = _dual_scene()
+    _keyhole(view, s.dims[0])
+    if keytf:
+        tf = dict(tf, keyhole_tables=_thin(configs.tf2d_bumps(99, tf["tables"]["alpha"].shape[0])))
+    march = dict(s.march, skip_rule=skip, term_T=1.0 / 32, precise=precise)
+    h1 = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    h2 = H.gpu_volume(gpu_ctx, v2, s.spacing)
+    g, (depth, _, _) = H.gpu_dual(gpu_ctx, h1, h2, tf, view, march, with_aux=True)
+    vd = configs.view_dict(view)
+    o32, _, od = orc.cast_dual(s.volume, v2, s.spacing, tf, vd, march, "f32")
+    o64, _, _ = orc.cast_dual(s.volume, v2, s.spacing, tf, vd, dict(march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, f"dual skip={skip} keytf={keytf} precise={precise}")
+    assert np.mean(depth == od) >= 0.999
+    assert (o32[..., 3] > 0).mean() > 0.1
+    if keytf:  # the keyhole TF shows inside the window
+        base, _, _ = orc.cast_dual(s.volume, v2, s.spacing, dict(tf, keyhole_tables=None), vd, march, "f32")
+        assert (np.abs(base.astype(int) - o32.astype(int)).max(axis=-1) > 1).sum() > 100
+    h1.close()
+    h2.close()
+
+
+def test_siblings_identical_across_cell_layouts(gpu_ctx):
+    s, view, v2, tf = _dual_scene()
+    _keyhole(view, s.dims[0])
+    tf2, inex = _inex_tf(s.tf2d["tables"], s.tf2d["params"])
+    march = dict(s.march, term_T=1.0 / 32)
+    outs = []
+    for layout in (abi.CELLS_PLAIN, abi.CELLS_BRICK4):
+        h1 = abi.Volume.from_array(gpu_ctx, s.volume, s.spacing, layout=layout)
+        h2 = abi.Volume.from_array(gpu_ctx, v2, s.spacing, layout=layout)
+        outs.append((H.gpu_2d_inex(gpu_ctx, h1, tf2, inex, True, view, march),
+                     H.gpu_dual(gpu_ctx, h1, h2, tf, view, march)))
+        h1.close()
+        h2.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_sibling_errors(gpu_ctx):
+    import torch
+    s, view, v2, tf = _dual_scene()
+    march = configs.march_params(s.march)
+    h1 = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    out = torch.zeros((view.height, view.width, 4), dtype=torch.uint8, device="cuda")
+    t = abi.TF2D(gpu_ctx, tf["tables"], configs.tf2d_params(tf["params"]))
+    with pytest.raises(abi.RvkError) as e:  # no in/ex table yet
+        abi.render_2d_inex(gpu_ctx, h1, t, view, march, out)
+    assert e.value.status == 5
+    with pytest.raises(ValueError):
+        t.set_inex(np.zeros((3, 3), np.float32))
+    t.set_inex(np.zeros((t.n, t.n), np.float32))
+    abi.render_2d_inex(gpu_ctx, h1, t, view, march, out)
+    t.set_inex(None)
+    with pytest.raises(abi.RvkError):
+        abi.render_2d_inex(gpu_ctx, h1, t, view, march, out)
+    small = abi.Volume.from_array(gpu_ctx, v2[:, :, :-2].copy(), s.spacing)
+    with pytest.raises(abi.RvkError) as e:  # component volumes of different sizes
+        abi.render_dual(gpu_ctx, h1, small, t, H.dual_params(tf["params"]), view, march, out)
+    assert e.value.status == 1
+    other = abi.Volume.from_array(gpu_ctx, v2.astype(np.int16), s.spacing)
+    with pytest.raises(abi.RvkError):  # different voxel types
+        abi.render_dual(gpu_ctx, h1, other, t, H.dual_params(tf["params"]), view, march, out)
+    torch.cuda.synchronize()
+    for o in (t, h1, small, other):
+        o.close()

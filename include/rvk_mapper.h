@@ -7,6 +7,8 @@
 //   Cuda1DVolumeMapper        <- V/vtkCuda1DVolumeMapper.h:30-64
 //   Cuda2DVolumeMapper        <- V/vtkCuda2DVolumeMapper.h:30-92
 //   CudaDRRImageVolumeMapper  <- V/vtkCudaDRRImageVolumeMapper.h:30-75
+//   Cuda2DInExLogicVolumeMapper <- V/vtkCuda2DInExLogicVolumeMapper.h:30-110
+//   CudaDualImageVolumeMapper <- V/vtkCudaDualImageVolumeMapper.h:30-95
 //   Cuda2DTransferFunction    <- C/vtkCuda2DTransferFunction.h
 //   FunctionPolygon(+Reader/Writer, the .2tf format) <- C/vtkCudaFunctionPolygon*.{h,cxx}
 // VTK is not available, so the VTK objects the mappers read (vtkImageData,
@@ -33,13 +35,14 @@
 namespace rvk {
 
 // ---- VTK stand-ins ------------------------------------------------------------
-struct ImageData {  // vtkImageData (point scalars, one component)
+struct ImageData {  // vtkImageData (point scalars)
   int dims[3] = {0, 0, 0};
   double spacing[3] = {1, 1, 1};
   double origin[3] = {0, 0, 0};
   int scalar_type = RVK_U8;  // rvk_dtype
+  int components = 1;        // interleaved per voxel; the dual-image mapper reads 2, the others 1
   const void* scalars = nullptr;  // borrowed, x fastest
-  void GetScalarRange(double range[2]) const;
+  void GetScalarRange(double range[2], int component = 0) const;
 };
 
 struct Plane {  // vtkPlane
@@ -264,9 +267,14 @@ class CudaVolumeMapper : public CudaObject {  // V/vtkCudaVolumeMapper.h:73-195
   void Deinitialize(bool withData) override;
   bool Fail(const std::string& what);
   bool Check(int rc, const char* what);
-  bool UploadFrame(int frame);
+  virtual bool UploadFrame(int frame);
+  virtual void DestroyVolumes();
+  // (Re)creates *v to hold frames like im; a change of size, type or spacing
+  // drops the other frames' inputs.
+  bool FitVolume(rvk_volume*& v, const ImageData& im, int frame);
   bool ComputeView(const Renderer& ren, const Volume& vol);
-  bool PostShade();
+  bool PostShade(bool inex = false);
+  bool FinishRGBA();  // copies the device RGBA8 image to output_
   rvk_volume* vol_ = nullptr;
   std::vector<ImageData> inputs_;
   std::vector<bool> has_input_;
@@ -330,6 +338,53 @@ class Cuda2DVolumeMapper : public CudaVolumeMapper {  // V/vtkCuda2DVolumeMapper
   int direct_n_ = 0;
   std::vector<std::vector<float>> direct_;
   rvk_tf2d_params direct_params_{};
+};
+
+// The 2D caster with in/ex logic.  Tables as the reference's handler builds
+// them (V/vtkCuda2DInExLogicTransferFunctionInformationHandler.cxx:137-217),
+// except that the gradient range is clipped by this input's estimate
+// (|scalar range| / min spacing) on every build, as the 2D mapper does; the
+// reference clips by the previous build's estimate (10 on the first).
+class Cuda2DInExLogicVolumeMapper : public CudaVolumeMapper {
+ public:
+  Cuda2DInExLogicVolumeMapper();
+  ~Cuda2DInExLogicVolumeMapper() override;
+  void SetVisualizationFunction(std::shared_ptr<Cuda2DTransferFunction> f) { fn_ = std::move(f); }
+  std::shared_ptr<Cuda2DTransferFunction> GetVisualizationFunction() const { return fn_; }
+  void SetInExLogicFunction(std::shared_ptr<Cuda2DTransferFunction> f) { inex_fn_ = std::move(f); }
+  std::shared_ptr<Cuda2DTransferFunction> GetInExLogicFunction() const { return inex_fn_; }
+  void SetUseBlackKeyhole(bool t) { black_ = t; }
+  bool GetUseBlackKeyhole() const { return black_; }
+  void SetTransferFunctionSize(int n) { tf_size_ = n; }
+
+ protected:
+  void InternalRender(const Renderer& ren, const Volume& vol) override;
+  std::shared_ptr<Cuda2DTransferFunction> fn_, inex_fn_;
+  bool black_ = false;
+  int tf_size_ = 512;  // V/vtkCuda2DInExLogicTransferFunctionInformationHandler.cxx:29
+};
+
+// The dual-image caster over a 2-component input: component 1 indexes the
+// table's intensity axis, component 2 its second axis (linear, no gradient),
+// shading uses component 1's gradient
+// (V/vtkCudaDualImageTransferFunctionInformationHandler.cxx:143-290).
+class CudaDualImageVolumeMapper : public CudaVolumeMapper {
+ public:
+  CudaDualImageVolumeMapper();
+  ~CudaDualImageVolumeMapper() override;
+  void SetFunction(std::shared_ptr<Cuda2DTransferFunction> f) { fn_ = std::move(f); }
+  std::shared_ptr<Cuda2DTransferFunction> GetFunction() const { return fn_; }
+  void SetKeyholeFunction(std::shared_ptr<Cuda2DTransferFunction> f) { kfn_ = std::move(f); }
+  std::shared_ptr<Cuda2DTransferFunction> GetKeyholeFunction() const { return kfn_; }
+  void SetTransferFunctionSize(int n) { tf_size_ = n; }
+
+ protected:
+  void InternalRender(const Renderer& ren, const Volume& vol) override;
+  bool UploadFrame(int frame) override;
+  void DestroyVolumes() override;
+  std::shared_ptr<Cuda2DTransferFunction> fn_, kfn_;
+  int tf_size_ = 512;  // V/vtkCudaDualImageTransferFunctionInformationHandler.cxx:30
+  rvk_volume* vol2_ = nullptr;  // component 2 (vol_ holds component 1)
 };
 
 class CudaDRRImageVolumeMapper : public CudaVolumeMapper {  // V/vtkCudaDRRImageVolumeMapper.cxx

@@ -15,12 +15,12 @@ namespace rvk {
 
 namespace {
 
-template <typename T> void range_of(const T* d, size_t n, double r[2]) {
+template <typename T> void range_of(const T* d, size_t n, double r[2], int stride = 1) {
   if (!n) { r[0] = r[1] = 0.0; return; }
   T lo = d[0], hi = d[0];
   for (size_t i = 1; i < n; ++i) {
-    lo = std::min(lo, d[i]);
-    hi = std::max(hi, d[i]);
+    lo = std::min(lo, d[i * stride]);
+    hi = std::max(hi, d[i * stride]);
   }
   r[0] = (double)lo;
   r[1] = (double)hi;
@@ -38,18 +38,19 @@ size_t dtype_bytes(int t) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-void ImageData::GetScalarRange(double r[2]) const {
+void ImageData::GetScalarRange(double r[2], int c) const {
   const size_t n = (size_t)dims[0] * dims[1] * dims[2];
   r[0] = r[1] = 0.0;
-  if (!scalars) return;
+  if (!scalars || c < 0 || c >= components) return;
+  const int k = components;
   switch (scalar_type) {
-    case RVK_U8: range_of((const uint8_t*)scalars, n, r); break;
-    case RVK_I8: range_of((const int8_t*)scalars, n, r); break;
-    case RVK_U16: range_of((const uint16_t*)scalars, n, r); break;
-    case RVK_I16: range_of((const int16_t*)scalars, n, r); break;
-    case RVK_U32: range_of((const uint32_t*)scalars, n, r); break;
-    case RVK_I32: range_of((const int32_t*)scalars, n, r); break;
-    case RVK_F32: range_of((const float*)scalars, n, r); break;
+    case RVK_U8: range_of((const uint8_t*)scalars + c, n, r, k); break;
+    case RVK_I8: range_of((const int8_t*)scalars + c, n, r, k); break;
+    case RVK_U16: range_of((const uint16_t*)scalars + c, n, r, k); break;
+    case RVK_I16: range_of((const int16_t*)scalars + c, n, r, k); break;
+    case RVK_U32: range_of((const uint32_t*)scalars + c, n, r, k); break;
+    case RVK_I32: range_of((const int32_t*)scalars + c, n, r, k); break;
+    case RVK_F32: range_of((const float*)scalars + c, n, r, k); break;
     default: break;
   }
 }
@@ -407,10 +408,14 @@ bool CudaVolumeMapper::Check(int rc, const char* what) {
   return Fail(std::string(what) + ": " + rvk_last_error_string());
 }
 
-void CudaVolumeMapper::Deinitialize(bool) {
-  if (!ctx_) return;
+void CudaVolumeMapper::DestroyVolumes() {
   if (vol_) rvk_volume_destroy(vol_);
   vol_ = nullptr;
+}
+
+void CudaVolumeMapper::Deinitialize(bool) {
+  if (!ctx_) return;
+  DestroyVolumes();
   if (d_out_) rvk_device_free(ctx_, d_out_);
   for (auto& a : d_aux_) {
     if (a) rvk_device_free(ctx_, a);
@@ -439,8 +444,7 @@ void CudaVolumeMapper::SetNumberOfFrames(int n) {
     return;
   }
   if (n == n_frames_) return;
-  if (vol_) rvk_volume_destroy(vol_);
-  vol_ = nullptr;
+  DestroyVolumes();
   inputs_.resize(n);
   has_input_.resize(n, false);
   n_frames_ = n;
@@ -449,26 +453,32 @@ void CudaVolumeMapper::SetNumberOfFrames(int n) {
     if (has_input_[f] && !UploadFrame(f)) return;
 }
 
-bool CudaVolumeMapper::UploadFrame(int f) {
-  const ImageData& im = inputs_[f];
-  if (!EnsureContext()) return Fail(ctx_error_);
-  if (vol_) {
+bool CudaVolumeMapper::FitVolume(rvk_volume*& v, const ImageData& im, int f) {
+  if (v) {
     int32_t d[3];
     double sp[3];
     int dt = 0, nf = 0;
-    rvk_volume_dims(vol_, d, sp, &dt, &nf);
+    rvk_volume_dims(v, d, sp, &dt, &nf);
     if (d[0] != im.dims[0] || d[1] != im.dims[1] || d[2] != im.dims[2] || dt != im.scalar_type || nf != n_frames_ ||
         sp[0] != im.spacing[0] || sp[1] != im.spacing[1] || sp[2] != im.spacing[2]) {
-      rvk_volume_destroy(vol_);
-      vol_ = nullptr;
+      rvk_volume_destroy(v);
+      v = nullptr;
       for (int g = 0; g < n_frames_; ++g)
         if (g != f) has_input_[g] = false;
     }
   }
-  if (!vol_) {
+  if (!v) {
     const int32_t d[3] = {im.dims[0], im.dims[1], im.dims[2]};
-    if (!Check(rvk_volume_create(ctx_, d, im.spacing, im.scalar_type, n_frames_, &vol_), "volume create")) return false;
+    return Check(rvk_volume_create(ctx_, d, im.spacing, im.scalar_type, n_frames_, &v), "volume create");
   }
+  return true;
+}
+
+bool CudaVolumeMapper::UploadFrame(int f) {
+  const ImageData& im = inputs_[f];
+  if (im.components != 1) return Fail("input must have 1 scalar component");
+  if (!EnsureContext()) return Fail(ctx_error_);
+  if (!FitVolume(vol_, im, f)) return false;
   return Check(rvk_volume_set_frame(vol_, f, im.scalars, 0), "volume upload");
 }
 
@@ -497,8 +507,7 @@ const ImageData* CudaVolumeMapper::GetInput(int frame) const {
 
 void CudaVolumeMapper::ClearInput() {
   std::fill(has_input_.begin(), has_input_.end(), false);
-  if (vol_) rvk_volume_destroy(vol_);
-  vol_ = nullptr;
+  DestroyVolumes();
 }
 
 void CudaVolumeMapper::ChangeFrame(int f) {
@@ -663,11 +672,21 @@ void CudaVolumeMapper::Render(const Renderer& ren, const Volume& vol) {
   InternalRender(ren, vol);
 }
 
-bool CudaVolumeMapper::PostShade() {
+bool CudaVolumeMapper::PostShade(bool inex) {
+  // with zero darkness both shades leave the image as it is
   if (!post_shade_ || (shade_.celr == 0.0f && shade_.disr == 0.0f)) return true;
-  return Check(rvk_post_shade(ctx_, view_.width, view_.height, &shade_, d_aux_[0], d_aux_[1], d_aux_[2],
-                              (uint8_t*)d_out_),
+  auto fn = inex ? rvk_post_shade_inex : rvk_post_shade;
+  return Check(fn(ctx_, view_.width, view_.height, &shade_, d_aux_[0], d_aux_[1], d_aux_[2], (uint8_t*)d_out_),
                "post shading");
+}
+
+bool CudaVolumeMapper::FinishRGBA() {
+  const size_t px = (size_t)view_.width * view_.height;
+  output_.width = view_.width;
+  output_.height = view_.height;
+  output_.values.clear();
+  output_.rgba.resize(px * 4);
+  return Check(rvk_copy_to_host(ctx_, output_.rgba.data(), d_out_, px * 4), "output copy");
 }
 
 // ---- 1D ---------------------------------------------------------------------------------
@@ -730,12 +749,7 @@ void Cuda1DVolumeMapper::InternalRender(const Renderer&, const Volume& vol) {
   }
   rvk_tf1d_destroy(tf);
   if (ok) ok = PostShade();
-  if (!ok) return;
-  output_.width = view_.width;
-  output_.height = view_.height;
-  output_.values.clear();
-  output_.rgba.resize(px * 4);
-  Check(rvk_copy_to_host(ctx_, output_.rgba.data(), d_out_, px * 4), "output copy");
+  if (ok) FinishRGBA();
 }
 
 // ---- 2D ---------------------------------------------------------------------------------
@@ -834,12 +848,154 @@ void Cuda2DVolumeMapper::InternalRender(const Renderer&, const Volume&) {
   }
   rvk_tf2d_destroy(tf);
   if (ok) ok = PostShade();
-  if (!ok) return;
-  output_.width = view_.width;
-  output_.height = view_.height;
-  output_.values.clear();
-  output_.rgba.resize(px * 4);
-  Check(rvk_copy_to_host(ctx_, output_.rgba.data(), d_out_, px * 4), "output copy");
+  if (ok) FinishRGBA();
+}
+
+// ---- 2D with in/ex logic ----------------------------------------------------------------
+Cuda2DInExLogicVolumeMapper::Cuda2DInExLogicVolumeMapper() : CudaVolumeMapper(1.0f / 32.0f) {}
+Cuda2DInExLogicVolumeMapper::~Cuda2DInExLogicVolumeMapper() = default;
+
+// V/vtkCuda2DInExLogicTransferFunctionInformationHandler.cxx:137-217
+void Cuda2DInExLogicVolumeMapper::InternalRender(const Renderer&, const Volume&) {
+  if (!fn_ || !inex_fn_) {
+    Fail("the visualization and in/ex logic functions must both be set");
+    return;
+  }
+  const ImageData& im = inputs_[frame_];
+  double sr[2];
+  im.GetScalarRange(sr);
+  const double minI = std::max(sr[0], (double)fn_->getMinIntensity());
+  const double maxI = std::min(sr[1], (double)fn_->getMaxIntensity());
+  const double highG =
+      std::fabs(sr[0] - sr[1]) / std::min(im.spacing[0], std::min(im.spacing[1], im.spacing[2]));
+  double minG = std::max({0.0, (double)fn_->getMinGradient(), (double)inex_fn_->getMinGradient()});
+  double maxG = std::min({highG, (double)fn_->getMaxGradient(), (double)inex_fn_->getMaxGradient()});
+  const double goff = maxG * 0.8;
+  maxG = (std::log(maxG * maxG + goff) - std::log(goff)) / std::log(2.0) + 1.0;
+  minG = (std::log(minG * minG + goff) - std::log(goff)) / std::log(2.0);
+  rvk_tf2d_params p;
+  p.intensity_low = (float)minI;
+  p.intensity_multiplier = (float)(1.0 / (maxI - minI));
+  p.gradient_multiplier = (float)(1.0 / (maxG - minG));
+  p.gradient_offset = (float)goff;
+  p.gradient_low = (float)(-minG - std::log(goff) / std::log(2.0));
+  const int n = tf_size_;
+  const size_t nn = (size_t)n * n;
+  std::vector<std::vector<float>> t(8, std::vector<float>(nn));
+  std::vector<float> inex(nn);
+  const float a[6] = {(float)minI, (float)maxI, 0.0f, (float)minG, (float)maxG, (float)goff};
+  fn_->GetTransferTable(t[1].data(), t[2].data(), t[3].data(), t[0].data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5],
+                        2);
+  fn_->GetShadingTable(t[4].data(), t[5].data(), t[6].data(), t[7].data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5],
+                       2);
+  inex_fn_->GetTransferTable(nullptr, nullptr, nullptr, inex.data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5], 2);
+  const float* tp[8];
+  for (int i = 0; i < 8; ++i) tp[i] = t[i].data();
+  rvk_tf2d* tf = nullptr;
+  if (!Check(rvk_tf2d_create(ctx_, n, tp, nullptr, &p, &tf), "2D transfer function upload")) return;
+  bool ok = Check(rvk_tf2d_set_inex(tf, inex.data()), "in/ex table upload");
+  const size_t px = (size_t)view_.width * view_.height;
+  if (ok) ok = EnsureOutput(px * 4, true);
+  if (ok) {
+    rvk_aux_out aux{d_aux_[0], d_aux_[1], d_aux_[2]};
+    ok = Check(rvk_render_2d_inex(ctx_, vol_, frame_, tf, &view_, &march_, black_, (uint8_t*)d_out_, &aux),
+               "in/ex render");
+  }
+  rvk_tf2d_destroy(tf);
+  if (ok) ok = PostShade(true);
+  if (ok) FinishRGBA();
+}
+
+// ---- dual image ----------------------------------------------------------------------------
+CudaDualImageVolumeMapper::CudaDualImageVolumeMapper() : CudaVolumeMapper(1.0f / 32.0f) {}
+CudaDualImageVolumeMapper::~CudaDualImageVolumeMapper() {
+  if (vol2_) rvk_volume_destroy(vol2_);
+  vol2_ = nullptr;
+}
+
+void CudaDualImageVolumeMapper::DestroyVolumes() {
+  CudaVolumeMapper::DestroyVolumes();
+  if (vol2_) rvk_volume_destroy(vol2_);
+  vol2_ = nullptr;
+}
+
+// V/vtkCudaDualImageVolumeMapper.cxx:76-160: a 2-component input, kept as one
+// device volume per component.
+bool CudaDualImageVolumeMapper::UploadFrame(int f) {
+  const ImageData& im = inputs_[f];
+  if (im.components != 2) return Fail("Input must have 2 components.");
+  if (!EnsureContext()) return Fail(ctx_error_);
+  if (!FitVolume(vol_, im, f) || !FitVolume(vol2_, im, f)) return false;
+  const size_t n = (size_t)im.dims[0] * im.dims[1] * im.dims[2], b = dtype_bytes(im.scalar_type);
+  std::vector<uint8_t> c1(n * b), c2(n * b);
+  const uint8_t* src = (const uint8_t*)im.scalars;
+  for (size_t i = 0; i < n; ++i) {
+    std::memcpy(&c1[i * b], src + 2 * i * b, b);
+    std::memcpy(&c2[i * b], src + (2 * i + 1) * b, b);
+  }
+  return Check(rvk_volume_set_frame(vol_, f, c1.data(), 0), "volume upload") &&
+         Check(rvk_volume_set_frame(vol2_, f, c2.data(), 0), "volume upload");
+}
+
+// V/vtkCudaDualImageTransferFunctionInformationHandler.cxx:143-290
+void CudaDualImageVolumeMapper::InternalRender(const Renderer&, const Volume&) {
+  if (!fn_ || fn_->GetNumberOfFunctionObjects() == 0) {
+    Fail("no 2D transfer function set");
+    return;
+  }
+  if (!vol2_) {
+    Fail("Render called without input data for the current frame");
+    return;
+  }
+  const bool key = kfn_ && kfn_->GetNumberOfFunctionObjects() > 0;
+  const ImageData& im = inputs_[frame_];
+  double sr[4];
+  im.GetScalarRange(sr, 0);
+  im.GetScalarRange(sr + 2, 1);
+  double fr[4] = {fn_->getMinIntensity(), fn_->getMaxIntensity(), fn_->getMinGradient(), fn_->getMaxGradient()};
+  if (key) {
+    fr[0] = std::min(fr[0], (double)kfn_->getMinIntensity());
+    fr[1] = std::max(fr[1], (double)kfn_->getMaxIntensity());
+    fr[2] = std::min(fr[2], (double)kfn_->getMinGradient());
+    fr[3] = std::max(fr[3], (double)kfn_->getMaxGradient());
+  }
+  const double lo1 = std::max(sr[0], fr[0]), hi1 = std::min(sr[1], fr[1]);
+  const double lo2 = std::max(sr[2], fr[2]), hi2 = std::min(sr[3], fr[3]);
+  const rvk_dual_params dp{(float)lo1, (float)(1.0 / (hi1 - lo1)), (float)lo2, (float)(1.0 / (hi2 - lo2))};
+  const int n = tf_size_;
+  const size_t nn = (size_t)n * n;
+  const float a[6] = {(float)lo1, (float)hi1, 0.0f, (float)lo2, (float)hi2, 0.0f};
+  std::vector<std::vector<float>> t(8, std::vector<float>(nn)), kt;
+  fn_->GetTransferTable(t[1].data(), t[2].data(), t[3].data(), t[0].data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5],
+                        0);
+  fn_->GetShadingTable(t[4].data(), t[5].data(), t[6].data(), t[7].data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5],
+                       0);
+  if (key) {
+    kt.assign(8, std::vector<float>(nn));
+    kfn_->GetTransferTable(kt[1].data(), kt[2].data(), kt[3].data(), kt[0].data(), n, n, a[0], a[1], a[2], a[3], a[4],
+                           a[5], 0);
+    kfn_->GetShadingTable(kt[4].data(), kt[5].data(), kt[6].data(), kt[7].data(), n, n, a[0], a[1], a[2], a[3], a[4],
+                          a[5], 0);
+  }
+  const float* tp[8];
+  const float* kp[8];
+  for (int i = 0; i < 8; ++i) {
+    tp[i] = t[i].data();
+    kp[i] = key ? kt[i].data() : nullptr;
+  }
+  const rvk_tf2d_params unused{};  // the dual caster takes its scalars from dp
+  rvk_tf2d* tf = nullptr;
+  if (!Check(rvk_tf2d_create(ctx_, n, tp, key ? kp : nullptr, &unused, &tf), "2D transfer function upload")) return;
+  const size_t px = (size_t)view_.width * view_.height;
+  bool ok = EnsureOutput(px * 4, true);
+  if (ok) {
+    rvk_aux_out aux{d_aux_[0], d_aux_[1], d_aux_[2]};
+    ok = Check(rvk_render_dual(ctx_, vol_, vol2_, frame_, tf, &dp, &view_, &march_, (uint8_t*)d_out_, &aux),
+               "dual-image render");
+  }
+  rvk_tf2d_destroy(tf);
+  if (ok) ok = PostShade();
+  if (ok) FinishRGBA();
 }
 
 // ---- DRR --------------------------------------------------------------------------------

@@ -237,6 +237,27 @@ void cpu_mapper_state() {
   m2.ChangeFrame(5);  // out of range: ignored
   ok &= m2.GetCurrentFrame() == 0;
   report("mapper_error_latch", ok, m2.GetLastError());
+
+  // component counts: the dual-image mapper takes 2, the others 1; both are
+  // refused before any device work
+  std::vector<uint8_t> vox(4 * 4 * 4 * 2, 7);
+  rvk::ImageData im;
+  im.dims[0] = im.dims[1] = im.dims[2] = 4;
+  im.scalars = vox.data();
+  rvk::CudaDualImageVolumeMapper dual;
+  dual.SetInputData(im);
+  ok = dual.HasError() && dual.GetLastError().find("2 components") != std::string::npos && !dual.GetInput();
+  im.components = 2;
+  rvk::Cuda2DInExLogicVolumeMapper inex;
+  inex.SetInputData(im);
+  ok &= inex.HasError() && inex.GetLastError().find("1 scalar component") != std::string::npos;
+  double r1[2], r2[2];
+  vox[1] = 9;
+  vox[2] = 3;
+  im.GetScalarRange(r1, 0);
+  im.GetScalarRange(r2, 1);
+  ok &= r1[0] == 3 && r1[1] == 7 && r2[0] == 7 && r2[1] == 9;
+  report("mapper_component_counts", ok, dual.GetLastError() + " / " + inex.GetLastError());
 }
 
 // ---------------------------------------------------------------------------------
@@ -620,6 +641,170 @@ void gpu_2d(Scene& s) {
   rvk_volume_destroy(vo);
 }
 
+// The 2D recipe's scalars (intensity range from the function and the input,
+// log-scaled gradient range clipped at |range| / min spacing) for a function
+// spanning intensities [i0, i1] and gradients [0, g1].
+rvk_tf2d_params recipe_2d(const rvk::ImageData& im, float i0, float i1, float g1, float out[6]) {
+  double sr[2];
+  im.GetScalarRange(sr);
+  const double minI = std::max(sr[0], (double)i0), maxI = std::min(sr[1], (double)i1);
+  const double highG = std::fabs(sr[0] - sr[1]) / 0.9;
+  double maxG = std::min(highG, (double)g1), minG = 0.0;
+  const double goff = maxG * 0.8;
+  maxG = (std::log(maxG * maxG + goff) - std::log(goff)) / std::log(2.0) + 1.0;
+  minG = (std::log(minG * minG + goff) - std::log(goff)) / std::log(2.0);
+  const float o[6] = {(float)minI, (float)maxI, 0.0f, (float)minG, (float)maxG, (float)goff};
+  std::memcpy(out, o, sizeof(o));
+  return rvk_tf2d_params{(float)minI, (float)(1.0 / (maxI - minI)), (float)(-minG - std::log(goff) / std::log(2.0)),
+                         (float)(1.0 / (maxG - minG)), (float)goff};
+}
+
+void gpu_inex(Scene& s) {
+  rvk::Cuda2DInExLogicVolumeMapper m;
+  m.SetInputData(s.im[0]);
+  auto fn = std::make_shared<rvk::Cuda2DTransferFunction>();
+  auto poly = rect(110, 250, 0, 400);
+  poly->SetColour(0.3f, 0.8f, 0.5f);
+  poly->SetOpacity(0.25f);
+  poly->SetAmbient(0.5f);
+  poly->SetDiffuse(0.5f);
+  poly->SetSpecularPower(1.0f);
+  fn->AddFunctionObject(poly);
+  // a transparent polygon widens the table's intensity range down to 80
+  fn->AddFunctionObject(rect(80, 110, 0, 400));
+  // in/ex: exclude rays that meet the bright core first, mark the shell
+  auto ie = std::make_shared<rvk::Cuda2DTransferFunction>();
+  auto core = rect(170, 250, 0, 400);
+  core->SetOpacity(1.0f);
+  auto shell = rect(90, 110, 0, 400);
+  shell->SetOpacity(0.3f);
+  ie->AddFunctionObject(core);
+  ie->AddFunctionObject(shell);
+  m.SetVisualizationFunction(fn);
+  m.SetInExLogicFunction(ie);
+  m.SetUseBlackKeyhole(true);
+  const int n = 128;
+  m.SetTransferFunctionSize(n);
+  m.SetPostShading(false);
+  rvk::Volume vol;
+  m.Render(s.ren, vol);
+  if (m.HasError()) {
+    report("inex_mapper_render", false, m.GetLastError());
+    return;
+  }
+  const rvk::ImageData& im = s.im[0];
+  float a[6];
+  const rvk_tf2d_params p = recipe_2d(im, 80, 250, 400, a);
+  std::vector<std::vector<float>> t(8, std::vector<float>(n * n));
+  std::vector<float> inex(n * n);
+  fn->GetTransferTable(t[1].data(), t[2].data(), t[3].data(), t[0].data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5], 2);
+  fn->GetShadingTable(t[4].data(), t[5].data(), t[6].data(), t[7].data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5], 2);
+  ie->GetTransferTable(nullptr, nullptr, nullptr, inex.data(), n, n, a[0], a[1], a[2], a[3], a[4], a[5], 2);
+  rvk_ctx* ctx = m.GetContext();
+  rvk_volume* vo = nullptr;
+  const int32_t d[3] = {im.dims[0], im.dims[1], im.dims[2]};
+  rvk_volume_create(ctx, d, im.spacing, RVK_U8, 1, &vo);
+  rvk_volume_set_frame(vo, 0, s.vox[0].data(), 0);
+  const float* tp[8];
+  for (int k = 0; k < 8; ++k) tp[k] = t[k].data();
+  rvk_tf2d* tf = nullptr;
+  rvk_tf2d_create(ctx, n, tp, nullptr, &p, &tf);
+  rvk_tf2d_set_inex(tf, inex.data());
+  const rvk_view& v = m.GetView();
+  const size_t px = (size_t)v.width * v.height;
+  DeviceImage out(ctx, px * 4);
+  rvk_march_params mp{1.0f, 1.0f / 32.0f, 1, 1, 0};
+  int rc = rvk_render_2d_inex(ctx, vo, 0, tf, &v, &mp, 1, (uint8_t*)out.p, nullptr);
+  std::string why;
+  const auto direct = out.get<uint8_t>();
+  bool ok = rc == RVK_OK && lsb_close(direct, m.GetOutput().rgba, why);
+  size_t black = 0, lit = 0;
+  for (size_t i = 0; i < direct.size(); i += 4) {
+    lit += direct[i + 3] > 0;
+    black += direct[i + 3] == 255 && direct[i] == 0 && direct[i + 1] == 0 && direct[i + 2] == 0;
+  }
+  ok &= lit > px / 50 && black > 0 && lit < px;
+  report("inex_mapper_polygon_vs_direct", ok, why + fmt(" lit %g black %g", (double)lit, (double)black));
+  rvk_tf2d_destroy(tf);
+  rvk_volume_destroy(vo);
+}
+
+void gpu_dual(Scene& s) {
+  const rvk::ImageData& im1 = s.im[0];
+  const size_t nv = s.vox[0].size();
+  std::vector<uint8_t> both(2 * nv);
+  for (size_t i = 0; i < nv; ++i) {
+    both[2 * i] = s.vox[0][i];
+    both[2 * i + 1] = s.vox[1][i];
+  }
+  rvk::ImageData im = im1;
+  im.components = 2;
+  im.scalars = both.data();
+  rvk::CudaDualImageVolumeMapper m;
+  m.SetInputData(im);
+  auto fn = std::make_shared<rvk::Cuda2DTransferFunction>();
+  auto poly = rect(60, 250, 60, 255);
+  poly->SetColour(0.9f, 0.4f, 0.7f);
+  poly->SetOpacity(0.2f);
+  poly->SetAmbient(0.6f);
+  poly->SetDiffuse(0.4f);
+  poly->SetSpecular(0.2f);
+  poly->SetSpecularPower(4.0f);
+  fn->AddFunctionObject(poly);
+  m.SetFunction(fn);
+  const int n = 128;
+  m.SetTransferFunctionSize(n);
+  m.SetPostShading(false);
+  rvk::Volume vol;
+  m.Render(s.ren, vol);
+  if (m.HasError()) {
+    report("dual_mapper_render", false, m.GetLastError());
+    return;
+  }
+  double sr[4];
+  im.GetScalarRange(sr, 0);
+  im.GetScalarRange(sr + 2, 1);
+  const double lo1 = std::max(sr[0], 60.0), hi1 = std::min(sr[1], 250.0);
+  const double lo2 = std::max(sr[2], 60.0), hi2 = std::min(sr[3], 255.0);
+  const rvk_dual_params dp{(float)lo1, (float)(1.0 / (hi1 - lo1)), (float)lo2, (float)(1.0 / (hi2 - lo2))};
+  std::vector<std::vector<float>> t(8, std::vector<float>(n * n));
+  fn->GetTransferTable(t[1].data(), t[2].data(), t[3].data(), t[0].data(), n, n, (float)lo1, (float)hi1, 0.0f,
+                       (float)lo2, (float)hi2, 0.0f, 0);
+  fn->GetShadingTable(t[4].data(), t[5].data(), t[6].data(), t[7].data(), n, n, (float)lo1, (float)hi1, 0.0f,
+                      (float)lo2, (float)hi2, 0.0f, 0);
+  rvk_ctx* ctx = m.GetContext();
+  rvk_volume *v1 = nullptr, *v2 = nullptr;
+  const int32_t d[3] = {im.dims[0], im.dims[1], im.dims[2]};
+  rvk_volume_create(ctx, d, im.spacing, RVK_U8, 1, &v1);
+  rvk_volume_create(ctx, d, im.spacing, RVK_U8, 1, &v2);
+  rvk_volume_set_frame(v1, 0, s.vox[0].data(), 0);
+  rvk_volume_set_frame(v2, 0, s.vox[1].data(), 0);
+  const float* tp[8];
+  for (int k = 0; k < 8; ++k) tp[k] = t[k].data();
+  const rvk_tf2d_params unused{};
+  rvk_tf2d* tf = nullptr;
+  rvk_tf2d_create(ctx, n, tp, nullptr, &unused, &tf);
+  const rvk_view& v = m.GetView();
+  const size_t px = (size_t)v.width * v.height;
+  DeviceImage out(ctx, px * 4);
+  rvk_march_params mp{1.0f, 1.0f / 32.0f, 1, 1, 0};
+  int rc = rvk_render_dual(ctx, v1, v2, 0, tf, &dp, &v, &mp, (uint8_t*)out.p, nullptr);
+  std::string why;
+  const auto direct = out.get<uint8_t>();
+  bool ok = rc == RVK_OK && lsb_close(direct, m.GetOutput().rgba, why);
+  size_t lit = 0;
+  for (size_t i = 3; i < direct.size(); i += 4) lit += direct[i] > 0;
+  ok &= lit > px / 50;
+  // swapping the components changes the picture (the axes are not symmetric)
+  int rc2 = rvk_render_dual(ctx, v2, v1, 0, tf, &dp, &v, &mp, (uint8_t*)out.p, nullptr);
+  std::string why2;
+  ok &= rc2 == RVK_OK && !lsb_close(out.get<uint8_t>(), direct, why2);
+  report("dual_mapper_polygon_vs_direct", ok, why + fmt(" lit %g", (double)lit));
+  rvk_tf2d_destroy(tf);
+  rvk_volume_destroy(v1);
+  rvk_volume_destroy(v2);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -641,6 +826,8 @@ int main(int argc, char** argv) {
     gpu_1d(s);
     gpu_planes(s);
     gpu_2d(s);
+    gpu_inex(s);
+    gpu_dual(s);
   } else {
     std::fprintf(stderr, "usage: %s cpu|gpu\n", argv[0]);
     return 2;

@@ -285,10 +285,14 @@ int rvk_drr_plan_destroy(rvk_drr_plan* plan);
  * captured graph stays valid (it reads the views from device memory). */
 int rvk_drr_plan_update_views(rvk_drr_plan* plan, const rvk_view* views, const float* dview);
 int rvk_drr_plan_offsets(const rvk_drr_plan* plan, int64_t* offsets /* n_views + 1 */);
+/* set_schedule applies to the render and to the registration forward;
+ * get_schedule reports the render's. */
 int rvk_drr_plan_set_schedule(rvk_drr_plan* plan, int schedule, int waves_per_simd);
 int rvk_drr_plan_get_schedule(const rvk_drr_plan* plan, int* schedule, int* waves_per_simd);
 /* Time every candidate schedule `reps` times on this plan (writes into out)
- * and keep the fastest; *best_ms (nullable) receives its time. */
+ * and keep the fastest; *best_ms (nullable) receives its time.  A plan with
+ * pose derivatives also tunes the registration step's forward (the march
+ * that produces the pose Jacobians) on its own. */
 int rvk_drr_plan_autotune(rvk_drr_plan* plan, float* out, int reps, float* best_ms);
 /* Render every view into the packed buffer `out` on the context stream. */
 int rvk_drr_plan_render(rvk_drr_plan* plan, float* out);

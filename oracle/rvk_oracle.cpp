@@ -131,16 +131,17 @@ template <typename VT> struct Grid {
 // a + t (b - a) with one rounding of the product-sum
 template <typename T> inline T lerp(T a, T b, T t) { return std::fma(t, b - a, a); }
 
-// Trilinear value at the half-texel-shifted coordinate h = p - 0.5 (the
-// marches carry h; sampling at p means passing p - 0.5).
-template <typename T, typename VT> inline T trilinear(const Grid<VT>& g, T hx, T hy, T hz) {
-  // clamping the coordinate to [-1, n] leaves clamp-addressed results unchanged
-  T x = std::fmin(std::fmax(hx, T(-1)), (T)g.nx);
-  T y = std::fmin(std::fmax(hy, T(-1)), (T)g.ny);
-  T z = std::fmin(std::fmax(hz, T(-1)), (T)g.nz);
+// Trilinear value at the cell coordinate c = p + 0.5 (the marches carry c;
+// sampling at p means passing p + 0.5).  Texel centres sit at i + 0.5, so the
+// corners along an axis are voxels floor(c) - 1 and floor(c), clamped.
+template <typename T, typename VT> inline T trilinear(const Grid<VT>& g, T cx, T cy, T cz) {
+  // clamping the coordinate to [0, n + 1] leaves clamp-addressed results unchanged
+  T x = std::fmin(std::fmax(cx, T(0)), (T)(g.nx + 1));
+  T y = std::fmin(std::fmax(cy, T(0)), (T)(g.ny + 1));
+  T z = std::fmin(std::fmax(cz, T(0)), (T)(g.nz + 1));
   T fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
   T ax = x - fx, ay = y - fy, az = z - fz;
-  int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
   int x0 = std::min(std::max(ix, 0), g.nx - 1), x1 = std::min(std::max(ix + 1, 0), g.nx - 1);
   int y0 = std::min(std::max(iy, 0), g.ny - 1), y1 = std::min(std::max(iy + 1, 0), g.ny - 1);
   int z0 = std::min(std::max(iz, 0), g.nz - 1), z1 = std::min(std::max(iz + 1, 0), g.nz - 1);
@@ -408,24 +409,23 @@ T drr_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_march* mp, const or
   int ee = k - fl2i_rd(r.exclude_end);
   ctl->ints(k, es, ee);
   const int kst = k;
-  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
-  // Sample coordinates are carried half-texel shifted (h = p - 0.5).
-  // Precise mode: h follows from the step index, (p0 - 0.5) + t inc with one
+  const V3<T> p0c{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
+  // Sample coordinates are carried as cell coordinates (c = p + 0.5).
+  // Precise mode: c follows from the step index, (p0 + 0.5) + t inc with one
   // rounding per axis (no drift from repeated p += inc); reference mode: the
-  // incrementally updated p, minus 0.5.
+  // incrementally updated p, plus 0.5.
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+    if (!mp->precise) return V3<T>{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
+    return V3<T>{std::fma(t, inc.x, p0c.x), std::fma(t, inc.y, p0c.y), std::fma(t, inc.z, p0c.z)};
   };
-  T comp = T(0);  // Kahan compensation of the line integral (precise mode)
+  // Precise mode sums the line integral in double, the reference mode in T.
+  double accd = (double)acc;
   auto add = [&](T v) {
-    if (!mp->precise) { acc += v; return; }
-    const T yk = v - comp;
-    const T tk = acc + yk;
-    comp = (tk - acc) - yk;
-    acc = tk;
+    if (mp->precise) accd += (double)v;
+    else acc += v;
   };
+  auto stop = [&]() { return mp->precise ? accd <= (double)stop_at : acc <= stop_at; };
   int a = 0, b = 0;
   while (k > 0) {
     const bool in_key = (es >= k && ee < k);
@@ -439,7 +439,7 @@ T drr_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_march* mp, const or
         if (a) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
         else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
         if (!a) add(alpha);
-        if (ctl->decide(dp->ref_termination && acc <= stop_at)) break;
+        if (ctl->decide(dp->ref_termination && stop())) break;
       } else {
         if (!a) { p.x += inc.x; p.y += inc.y; p.z += inc.z; k--; }
         b = !a;
@@ -450,10 +450,11 @@ T drr_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_march* mp, const or
       p.x += inc.x; p.y += inc.y; p.z += inc.z; k--;
       if (ctl->decide(alpha < T(0) && !in_key)) {
         add(alpha);
-        if (ctl->decide(dp->ref_termination && acc <= stop_at)) break;
+        if (ctl->decide(dp->ref_termination && stop())) break;
       }
     }
   }
+  if (mp->precise) acc = (T)accd;
   if (ref_mode) {
     T out = T(256) - std::exp(acc);
     return out < T(0) ? T(0) : out;
@@ -527,15 +528,15 @@ RGBAT<T> cast1d_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_tf1d* tf,
   ctl->ints(k, es, ee);
   k0 = k;
   const int kst = k;
-  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
-  // Sample coordinates are carried half-texel shifted (h = p - 0.5).
-  // Precise mode: h follows from the step index, (p0 - 0.5) + t inc with one
+  const V3<T> p0c{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
+  // Sample coordinates are carried as cell coordinates (c = p + 0.5).
+  // Precise mode: c follows from the step index, (p0 + 0.5) + t inc with one
   // rounding per axis (no drift from repeated p += inc); reference mode: the
-  // incrementally updated p, minus 0.5.
+  // incrementally updated p, plus 0.5.
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+    if (!mp->precise) return V3<T>{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
+    return V3<T>{std::fma(t, inc.x, p0c.x), std::fma(t, inc.y, p0c.y), std::fma(t, inc.z, p0c.z)};
   };
   int a = 0, b = 0;
   auto accumulate = [&](T alpha, T u) {
@@ -624,15 +625,15 @@ RGBAT<T> cast2d_ray(const Grid<VT>& g, const Grid<uint8_t>* gm_grid, const VolIn
   ctl->ints(k, es, ee);
   const int k0 = k;
   const int kst = k;
-  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
-  // Sample coordinates are carried half-texel shifted (h = p - 0.5).
-  // Precise mode: h follows from the step index, (p0 - 0.5) + t inc with one
+  const V3<T> p0c{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
+  // Sample coordinates are carried as cell coordinates (c = p + 0.5).
+  // Precise mode: c follows from the step index, (p0 + 0.5) + t inc with one
   // rounding per axis (no drift from repeated p += inc); reference mode: the
-  // incrementally updated p, minus 0.5.
+  // incrementally updated p, plus 0.5.
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+    if (!mp->precise) return V3<T>{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
+    return V3<T>{std::fma(t, inc.x, p0c.x), std::fma(t, inc.y, p0c.y), std::fma(t, inc.z, p0c.z)};
   };
   int a = 0, b = 0;
   while (k > 0) {
@@ -812,11 +813,11 @@ RGBAT<T> inex_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_tf2d* tf, c
   const V3<T>& inc = r.inc;
   const T sx = vi.sp[0], sy = vi.sp[1], sz = vi.sp[2];
   const T ray_len = std::sqrt(inc.x * inc.x * sx * sx + inc.y * inc.y * sy * sy + inc.z * inc.z * sz * sz);
-  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+  const V3<T> p0c{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+    if (!mp->precise) return V3<T>{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
+    return V3<T>{std::fma(t, inc.x, p0c.x), std::fma(t, inc.y, p0c.y), std::fma(t, inc.z, p0c.z)};
   };
   bool skip = false, back = false, special = false;
   while (k > 0) {
@@ -910,11 +911,11 @@ RGBAT<T> dual_ray(const Grid<VT>& g1, const Grid<VT>& g2, const VolInfo<T>& vi, 
   const V3<T>& inc = r.inc;
   const T sx = vi.sp[0], sy = vi.sp[1], sz = vi.sp[2];
   const T ray_len = std::sqrt(inc.x * inc.x * sx * sx + inc.y * inc.y * sy * sy + inc.z * inc.z * sz * sz);
-  const V3<T> p0h{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+  const V3<T> p0c{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
   auto cur = [&]() -> V3<T> {
-    if (!mp->precise) return V3<T>{p.x - T(0.5), p.y - T(0.5), p.z - T(0.5)};
+    if (!mp->precise) return V3<T>{p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
     const T t = (T)(kst - k);
-    return V3<T>{std::fma(t, inc.x, p0h.x), std::fma(t, inc.y, p0h.y), std::fma(t, inc.z, p0h.z)};
+    return V3<T>{std::fma(t, inc.x, p0c.x), std::fma(t, inc.y, p0c.y), std::fma(t, inc.z, p0c.z)};
   };
   int a = 0, b = 0;
   while (k > 0) {
@@ -1219,12 +1220,12 @@ long long orc_count_samples(const orc_volume* vol, const orc_view* view, const o
 
 double orc_sample_f64(const orc_volume* vol, double px, double py, double pz) {
   double out = 0.0;
-  with_grid(vol, [&](auto g) { out = trilinear<double>(g, px - 0.5, py - 0.5, pz - 0.5); return 0; });
+  with_grid(vol, [&](auto g) { out = trilinear<double>(g, px + 0.5, py + 0.5, pz + 0.5); return 0; });
   return out;
 }
 float orc_sample_f32(const orc_volume* vol, float px, float py, float pz) {
   float out = 0.0f;
-  with_grid(vol, [&](auto g) { out = trilinear<float>(g, px - 0.5f, py - 0.5f, pz - 0.5f); return 0; });
+  with_grid(vol, [&](auto g) { out = trilinear<float>(g, px + 0.5f, py + 0.5f, pz + 0.5f); return 0; });
   return out;
 }
 

@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void jac_grad_kernel(JacGradArgs a) {
 // every detector, intersects the covered rays with the support, and re-marches
 // only those samples; one thread owns one voxel, so there are no atomics.
 struct RayRec {
-  float p0x, p0y, p0z, ix, iy, iz;  // shifted start p0 - 0.5 and step
+  float p0x, p0y, p0z, ix, iy, iz;  // start in cell coordinates (p0 + 0.5) and step
   int n;
   float cs, ci;   // alpha = cs * value + ci
   float coef;     // d loss / d L_p * ds
@@ -397,9 +397,9 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
   const float ds = sqrtf(r.inc.x * r.inc.x * g.sp[0] * g.sp[0] + r.inc.y * r.inc.y * g.sp[1] * g.sp[1] +
                          r.inc.z * r.inc.z * g.sp[2] * g.sp[2]);
   RayRec rr;
-  rr.p0x = (r.start.x + j * r.inc.x) - 0.5f;
-  rr.p0y = (r.start.y + j * r.inc.y) - 0.5f;
-  rr.p0z = (r.start.z + j * r.inc.z) - 0.5f;
+  rr.p0x = (r.start.x + j * r.inc.x) + 0.5f;
+  rr.p0y = (r.start.y + j * r.inc.y) + 0.5f;
+  rr.p0z = (r.start.z + j * r.inc.z) + 0.5f;
   rr.ix = r.inc.x; rr.iy = r.inc.y; rr.iz = r.inc.z;
   rr.n = step_count(r.num_steps - j);
   rr.ci = a.b.m.intercept * -ds;
@@ -440,9 +440,10 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
   float acc = 0.0f;
   if (live) {
     // the support in sample coordinates p: (i-0.5, i+1.5)^3; lo is its low
-    // corner, hl the same corner in shifted coordinates h = p - 0.5
+    // corner, cl the same corner in cell coordinates c = p + 0.5, where the
+    // voxel's centre is at i + 1
     const float lo[3] = {(float)i - 0.5f, (float)jj - 0.5f, (float)k - 0.5f};
-    const float hl[3] = {(float)i - 1.0f, (float)jj - 1.0f, (float)k - 1.0f};
+    const float cl[3] = {(float)i, (float)jj, (float)k};
     for (int v = 0; v < a.b.n_views; ++v) {
       const BatchView& bv = a.b.views[v];
       const int W = bv.v.W, H = bv.v.H;
@@ -472,16 +473,16 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
         for (int px = x0; px <= x1; ++px) {
           const RayRec rr = a.rays[bv.pix_off + (long long)py * W + px];
           if (rr.n <= 0 || rr.coef == 0.0f) continue;
-          // sample indices m with h0 + m inc inside the open support box
+          // sample indices m with c0 + m inc inside the open support box
           float tlo = 0.0f, thi = (float)(rr.n - 1);
           const float p0[3] = {rr.p0x, rr.p0y, rr.p0z}, in[3] = {rr.ix, rr.iy, rr.iz};
 #pragma unroll
           for (int ax = 0; ax < 3; ++ax) {
             if (in[ax] != 0.0f) {
-              const float ta = (hl[ax] - p0[ax]) / in[ax], tb = (hl[ax] + 2.0f - p0[ax]) / in[ax];
+              const float ta = (cl[ax] - p0[ax]) / in[ax], tb = (cl[ax] + 2.0f - p0[ax]) / in[ax];
               tlo = fmaxf(tlo, fminf(ta, tb));
               thi = fminf(thi, fmaxf(ta, tb));
-            } else if (p0[ax] <= hl[ax] || p0[ax] >= hl[ax] + 2.0f) {
+            } else if (p0[ax] <= cl[ax] || p0[ax] >= cl[ax] + 2.0f) {
               thi = -1.0f;
             }
           }
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
           for (int m = m0; m <= m1; ++m) {
             const float t = (float)m;
             const float qx = fmaf(t, rr.ix, rr.p0x), qy = fmaf(t, rr.iy, rr.p0y), qz = fmaf(t, rr.iz, rr.p0z);
-            const float w = hat(qx - (float)i) * hat(qy - (float)jj) * hat(qz - (float)k);
+            const float w = hat(qx - (cl[0] + 1.0f)) * hat(qy - (cl[1] + 1.0f)) * hat(qz - (cl[2] + 1.0f));
             if (w <= 0.0f) continue;
             const float alpha = fmaf(rr.cs, trilinear(vol, g, qx, qy, qz), rr.ci);
             if (alpha < 0.0f) acc += rr.coef * w;
@@ -524,6 +525,10 @@ struct rvk_drr_plan {
   float* d_jac = nullptr;        // 6 x total pixels, allocated with the plan when it has pose derivatives
   int schedule = RVK_SCHED_GRID;
   int waves_per_simd = 4;
+  // the fused registration forward (pose Jacobians) has its own schedule:
+  // its per-wave work differs enough that the best choice can differ
+  int jac_schedule = RVK_SCHED_GRID;
+  int jac_waves_per_simd = 4;
   int n_cu = 256;
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -607,9 +612,11 @@ int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s, bool jac = false)
   const bool skip = p->march.skip_rule != 0, pr = p->march.precise != 0;
   return dispatch_cells(p->vol->dtype, p->vol->layout, [&](auto tag) {
     using CV = decltype(tag);
-    if (p->schedule == RVK_SCHED_PERSISTENT) {
+    const int sched = jac ? p->jac_schedule : p->schedule;
+    const int wps = jac ? p->jac_waves_per_simd : p->waves_per_simd;
+    if (sched == RVK_SCHED_PERSISTENT) {
       hipLaunchKernelGGL(reset_queues_kernel, dim3(1), dim3(64), 0, s, p->d_queues);
-      const int blocks = std::max(1, std::min(p->n_cu * p->waves_per_simd, (p->n_items + 3) / 4));
+      const int blocks = std::max(1, std::min(p->n_cu * wps, (p->n_items + 3) / 4));
 #define RVK_LP(S, P, J) hipLaunchKernelGGL((batch_persistent_kernel<CV, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
       if (jac) { if (skip) RVK_LP(true, true, true); else RVK_LP(false, true, true); }
       else if (skip) { if (pr) RVK_LP(true, true, false); else RVK_LP(true, false, false); }
@@ -792,9 +799,11 @@ int rvk_drr_plan_set_schedule(rvk_drr_plan* p, int schedule, int wps) {
   if (schedule != RVK_SCHED_GRID && schedule != RVK_SCHED_PERSISTENT)
     return fail(RVK_ERR_INVALID_ARGUMENT, "unknown schedule %d", schedule);
   if (wps < 1 || wps > 8) return fail(RVK_ERR_INVALID_ARGUMENT, "waves_per_simd must be within 1..8");
-  if (schedule != p->schedule || wps != p->waves_per_simd) drop_graph(p);
-  p->schedule = schedule;
-  p->waves_per_simd = wps;
+  if (schedule != p->schedule || wps != p->waves_per_simd || schedule != p->jac_schedule ||
+      wps != p->jac_waves_per_simd)
+    drop_graph(p);
+  p->schedule = p->jac_schedule = schedule;
+  p->waves_per_simd = p->jac_waves_per_simd = wps;
   return RVK_OK;
 }
 
@@ -815,28 +824,34 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
   hipEvent_t e0, e1;
   RVK_HIP(hipEventCreate(&e0));
   RVK_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  int bs = p->schedule, bw = p->waves_per_simd;
   int rc = RVK_OK;
-  for (const auto& c : cand) {
-    p->schedule = c[0];
-    p->waves_per_simd = c[1];
-    rc = launch_forward(p, out, p->ctx->stream);  // warm-up
-    if (rc) break;
-    hipEventRecord(e0, p->ctx->stream);
-    for (int r = 0; r < reps && !rc; ++r) rc = launch_forward(p, out, p->ctx->stream);
-    hipEventRecord(e1, p->ctx->stream);
-    hipEventSynchronize(e1);
-    float ms = 0.0f;
-    hipEventElapsedTime(&ms, e0, e1);
-    ms /= reps;
-    if (ms < best) { best = ms; bs = c[0]; bw = c[1]; }
+  // the plain render, and for plans with pose derivatives the registration
+  // forward, each keep their fastest candidate
+  for (int jac = 0; jac < ((p->d_jac && p->march.precise) ? 2 : 1) && !rc; ++jac) {
+    int& sched = jac ? p->jac_schedule : p->schedule;
+    int& wps = jac ? p->jac_waves_per_simd : p->waves_per_simd;
+    float best = 1e30f;
+    int bs = sched, bw = wps;
+    for (const auto& c : cand) {
+      sched = c[0];
+      wps = c[1];
+      rc = launch_forward(p, out, p->ctx->stream, jac);  // warm-up
+      if (rc) break;
+      hipEventRecord(e0, p->ctx->stream);
+      for (int r = 0; r < reps && !rc; ++r) rc = launch_forward(p, out, p->ctx->stream, jac);
+      hipEventRecord(e1, p->ctx->stream);
+      hipEventSynchronize(e1);
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      ms /= reps;
+      if (ms < best) { best = ms; bs = c[0]; bw = c[1]; }
+    }
+    sched = bs;
+    wps = bw;
+    if (!jac && best_ms) *best_ms = best;
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  p->schedule = bs;
-  p->waves_per_simd = bw;
-  if (best_ms) *best_ms = best;
   return rc;
 }
 

@@ -109,14 +109,14 @@ __global__ __launch_bounds__(256) void cast1d_kernel(Cast1dArgs a) {
   const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
   const int es = (int)ceilf(r.exclude_start), ee = (int)floorf(r.exclude_end);
   const int kst = k;
-  const V3<float> p0h = half_shift(p);
+  const V3<float> p0c = cell_coord(p);
   int sa = 0, sb = 0;
 
   auto accumulate = [&](float alpha, float u) {
     float shd, shs;
     if (GRAD) {
       V3<float> gr;
-      const V3<float> q = sample_pos<PRECISE>(p, p0h, inc, kst, k);
+      const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
       gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
       gr.y = (trilinear(vol, g, q.x, q.y + 1.0f, q.z) - trilinear(vol, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
       gr.z = (trilinear(vol, g, q.x, q.y, q.z + 1.0f) - trilinear(vol, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void cast1d_kernel(Cast1dArgs a) {
       sb = 0;
       continue;
     }
-    const V3<float> q = sample_pos<PRECISE>(p, p0h, inc, kst, k);
+    const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
     const float u = mult * (trilinear(vol, g, q.x, q.y, q.z) - lo);
     const float alpha = tf1d_lookup(t_alpha, n, u);
     if (SKIP) {
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
   es = k - es;
   ee = k - ee;
   const int kst = k;
-  const V3<float> p0h = half_shift(p);
+  const V3<float> p0c = cell_coord(p);
   int sa = 0, sb = 0;
   while (k > 0) {
     const bool in_key = (es >= k && ee < k);
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
       sb = 0;
       continue;
     }
-    const V3<float> q = sample_pos<PRECISE>(p, p0h, inc, kst, k);
+    const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
     const float u = mult * (trilinear(vol, g, q.x, q.y, q.z) - lo);
     V3<float> gr{0.0f, 0.0f, 0.0f};
     float gidx, gmag;

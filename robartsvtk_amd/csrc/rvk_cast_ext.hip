@@ -76,10 +76,10 @@ __global__ __launch_bounds__(256) void inex_kernel(InExArgs a) {
   const float rlen = ray_length(g, inc);
   const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
   const int kst = k;
-  const V3<float> p0h = half_shift(p);
+  const V3<float> p0c = cell_coord(p);
   bool skip = false, back = false, special = false;
   while (k > 0) {
-    const V3<float> q = sample_pos<PRECISE>(p, p0h, inc, kst, k);
+    const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
     const float u = mult * (trilinear(vol, g, q.x, q.y, q.z) - lo);
     V3<float> gr;
     gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
   const float rlen = ray_length(g, inc);
   const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
   const int kst = k;
-  const V3<float> p0h = half_shift(p);
+  const V3<float> p0c = cell_coord(p);
   int sa = 0, sb = 0;
   while (k > 0) {
     const bool in_key = (es >= k && ee < k);
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
       sb = 0;
       continue;
     }
-    const V3<float> q = sample_pos<PRECISE>(p, p0h, inc, kst, k);
+    const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
     const float u1 = m1 * (trilinear(vol1, g, q.x, q.y, q.z) - lo1);
     const float u2 = m2 * (trilinear(vol2, g, q.x, q.y, q.z) - lo2);
     V3<float> gr;

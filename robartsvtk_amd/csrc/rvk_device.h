@@ -205,40 +205,43 @@ __device__ __forceinline__ float load_voxel(Cells<T, L> v, const VolGeom& g, int
 }
 
 // Cell and weights of tex3D(unnormalised, linear, clamp) at p, given the
-// half-texel-shifted coordinate h = p - 0.5: trilinear at h with indices
-// clamped to [0, n-1].  Clamping h to [-1, n] first leaves every
-// clamp-addressed result unchanged and keeps the integer conversion in range
-// (sample coordinates are finite: rays with non-finite steps take no samples).
+// cell coordinate c = p + 0.5.  Texel centres sit at i + 0.5, so the
+// reconstruction at p interpolates voxels floor(p - 0.5) and the next one
+// along each axis: the corners of cell floor(c), since cell i holds voxels
+// i - 1 and i (clamped).  Clamping c to [0, n + 1] leaves every
+// clamp-addressed result unchanged; on c >= 0 truncation is floor and the
+// fraction c - floor(c) is exact, so the cell is one v_cvt_i32_f32 and the
+// weight one v_fract_f32 per axis (sample coordinates are finite: rays with
+// non-finite steps take no samples).
 struct CellPos {
   uint32_t cell;
   float ax, ay, az;
   bool xs, ys, zs;  // the two corners along the axis are distinct voxels
 };
 template <int L>
-__device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float hx, float hy, float hz) {
-  const float x = __builtin_amdgcn_fmed3f(hx, -1.0f, (float)g.n[0]);
-  const float y = __builtin_amdgcn_fmed3f(hy, -1.0f, (float)g.n[1]);
-  const float z = __builtin_amdgcn_fmed3f(hz, -1.0f, (float)g.n[2]);
-  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+__device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float cx, float cy, float cz) {
+  const float x = __builtin_amdgcn_fmed3f(cx, 0.0f, (float)(g.n[0] + 1));
+  const float y = __builtin_amdgcn_fmed3f(cy, 0.0f, (float)(g.n[1] + 1));
+  const float z = __builtin_amdgcn_fmed3f(cz, 0.0f, (float)(g.n[2] + 1));
   CellPos c;
-  c.ax = x - fx;
-  c.ay = y - fy;
-  c.az = z - fz;
-  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-  c.cell = CellOrder<L>::index(g, ix + 1, iy + 1, iz + 1);
-  c.xs = ix >= 0 && ix < g.n[0] - 1;
-  c.ys = iy >= 0 && iy < g.n[1] - 1;
-  c.zs = iz >= 0 && iz < g.n[2] - 1;
+  c.ax = __builtin_amdgcn_fractf(x);
+  c.ay = __builtin_amdgcn_fractf(y);
+  c.az = __builtin_amdgcn_fractf(z);
+  const int ix = (int)x, iy = (int)y, iz = (int)z;
+  c.cell = CellOrder<L>::index(g, ix, iy, iz);
+  c.xs = ix >= 1 && ix < g.n[0];
+  c.ys = iy >= 1 && iy < g.n[1];
+  c.zs = iz >= 1 && iz < g.n[2];
   return c;
 }
 
 // a + t (b - a) as one rounding of the product-sum
 __device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
 
-// Trilinear value at the shifted coordinate h = p - 0.5.
+// Trilinear value at the cell coordinate c = p + 0.5.
 template <typename T, int L>
-__device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, float hx, float hy, float hz) {
-  const CellPos c = cell_pos<L>(g, hx, hy, hz);
+__device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, float cx, float cy, float cz) {
+  const CellPos c = cell_pos<L>(g, cx, cy, cz);
   float v[8];
   CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
@@ -248,16 +251,16 @@ __device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, fl
 }
 
 // Trilinear value and its gradient with respect to p (voxel index units) on
-// the same clamped cell, at the shifted coordinate h = p - 0.5.  The value is
+// the same clamped cell, at the cell coordinate c = p + 0.5.  The value is
 // computed with exactly the operations of trilinear(), so branch decisions
 // made on it match the plain caster.
 // d/dx = lerp_z(lerp_y(v100-v000, v110-v010), lerp_y(v101-v001, v111-v011)),
 // d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0; axes whose two corners
 // clamp to one voxel get a zero derivative.
 template <typename T, int L>
-__device__ __forceinline__ float trilinear_vg(Cells<T, L> vol, const VolGeom& g, float hx, float hy, float hz,
+__device__ __forceinline__ float trilinear_vg(Cells<T, L> vol, const VolGeom& g, float cx, float cy, float cz,
                                               V3<float>& gr) {
-  const CellPos c = cell_pos<L>(g, hx, hy, hz);
+  const CellPos c = cell_pos<L>(g, cx, cy, cz);
   float v[8];
   CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);

@@ -2,6 +2,8 @@
 // caster and by the batched forward / adjoint kernels, so every path runs the
 // same operations in the same order.
 #pragma once
+#include <type_traits>
+
 #include "rvk_internal.h"
 
 namespace rvk {
@@ -55,21 +57,25 @@ __device__ __forceinline__ void step(V3<float>& p, const V3<float>& inc) {
   p.x += inc.x; p.y += inc.y; p.z += inc.z;
 }
 
-// Sample coordinate, shifted by the half texel (h = p - 0.5, what trilinear()
-// takes).  Reference mode keeps the incrementally updated p (p += inc per
-// step, as the CUDA kernels do) and subtracts 0.5; PRECISE derives it from
-// the step index, h = (p0 - 0.5) + (k_start - k) inc with one rounding per
-// axis, which removes the drift repeated FP32 additions build up.
+// Sample coordinate in cell units (c = p + 0.5, what trilinear() takes).
+// Reference mode keeps the incrementally updated p (p += inc per step, as the
+// CUDA kernels do) and adds 0.5; PRECISE derives it from the step index,
+// c = (p0 + 0.5) + (k_start - k) inc with one rounding per axis, which removes
+// the drift repeated FP32 additions build up.
 template <bool PRECISE>
-__device__ __forceinline__ V3<float> sample_pos(const V3<float>& p, const V3<float>& p0h, const V3<float>& inc, int kst,
+__device__ __forceinline__ V3<float> sample_pos(const V3<float>& p, const V3<float>& p0c, const V3<float>& inc,
+                                                float t) {
+  if (!PRECISE) return V3<float>{p.x + 0.5f, p.y + 0.5f, p.z + 0.5f};
+  return V3<float>{fmaf(t, inc.x, p0c.x), fmaf(t, inc.y, p0c.y), fmaf(t, inc.z, p0c.z)};
+}
+template <bool PRECISE>
+__device__ __forceinline__ V3<float> sample_pos(const V3<float>& p, const V3<float>& p0c, const V3<float>& inc, int kst,
                                                 int k) {
-  if (!PRECISE) return V3<float>{p.x - 0.5f, p.y - 0.5f, p.z - 0.5f};
-  const float t = (float)(kst - k);
-  return V3<float>{fmaf(t, inc.x, p0h.x), fmaf(t, inc.y, p0h.y), fmaf(t, inc.z, p0h.z)};
+  return sample_pos<PRECISE>(p, p0c, inc, (float)(kst - k));
 }
 
-__device__ __forceinline__ V3<float> half_shift(const V3<float>& p) {
-  return V3<float>{p.x - 0.5f, p.y - 0.5f, p.z - 0.5f};
+__device__ __forceinline__ V3<float> cell_coord(const V3<float>& p) {
+  return V3<float>{p.x + 0.5f, p.y + 0.5f, p.z + 0.5f};
 }
 
 // ===========================================================================
@@ -84,76 +90,77 @@ struct NoVisit {
 };
 
 // Line integral along one ray.  The accumulator starts at ln(256*offset) in
-// the reference RGBA8 mode and at 0 in the float modes.  `visit(h, t, alpha,
-// grad)` is called for every accumulated sample, with h its shifted sample
-// coordinate (h = p0 - 0.5 + t*inc in PRECISE mode); with VG the trilinear
-// gradient at h comes from the same gathers as the value (the adjoint marches
+// the reference RGBA8 mode and at 0 in the float modes.  `visit(c, t, alpha,
+// grad)` is called for every accumulated sample, with c its sample cell
+// coordinate (c = p0 + 0.5 + t*inc in PRECISE mode); with VG the trilinear
+// gradient at c comes from the same gathers as the value (the adjoint marches
 // through this very control flow).
 template <typename CV, bool SKIP, bool PRECISE, bool VG = false, typename F = NoVisit>
 __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMarchArgs& a,
                                            const Ray& r, int x, int y, F&& visit = F()) {
   const bool ref_mode = a.output == RVK_DRR_RGBA8_REF;
   const float lg = logf(256.0f * a.offset);
-  float acc = ref_mode ? lg : 0.0f;
+  // PRECISE sums the line integral in double: a converted f32 sample and one
+  // f64 add cost less than a compensated f32 sum and carry more precision
+  // (the f64 add runs at the f32 rate).  The reference mode sums in f32.
+  using Acc = typename std::conditional<PRECISE, double, float>::type;
+  Acc acc = ref_mode ? (Acc)lg : (Acc)0;
   // termination at acc <= stop_at only with ref_term (otherwise never)
-  const float stop_at = !a.ref_term ? -__builtin_huge_valf() : (ref_mode ? 0.0f : -lg);
+  const Acc stop_at = !a.ref_term ? (Acc)-__builtin_huge_valf() : (ref_mode ? (Acc)0 : (Acc)-lg);
   const float j = ray_offset(a.jitter, x, y);
-  int k = step_count(r.num_steps - j);
+  const int kst = step_count(r.num_steps - j);
   V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
   const V3<float> inc = r.inc;
   const float ds = ray_length(g, inc);
   const float ci = a.intercept * -ds;
   const float cs = a.slope * -ds;
-  const int es = k - (int)ceilf(r.exclude_start);
-  const int ee = k - (int)floorf(r.exclude_end);
-  const int kst = k;
-  const V3<float> p0h = half_shift(p);
-  float comp = 0.0f;  // Kahan compensation of the line integral (PRECISE)
-  auto add = [&](float v) {
-    if (!PRECISE) { acc += v; return; }
-    const float yk = v - comp;
-    const float tk = acc + yk;
-    comp = (tk - acc) - yk;
-    acc = tk;
-  };
+  // The march counts in t = kst - k (k = steps left), exactly, as a float:
+  // sample t sits at p0 + t inc, the ray ends at t = kst, and the keyhole
+  // window es >= k > ee is t in [kst - es, kst - ee).
+  const int es = kst - (int)ceilf(r.exclude_start);
+  const int ee = kst - (int)floorf(r.exclude_end);
+  const float tend = (float)kst, kw0 = (float)(kst - es), kw1 = (float)(kst - ee);
+  const V3<float> p0c = cell_coord(p);
+  float t = 0.0f;
   int sa = 0, sb = 0;
-  while (k > 0) {
-    const bool in_key = (es >= k && ee < k);
-    const V3<float> q = sample_pos<PRECISE>(p, p0h, inc, kst, k);
+  while (t < tend) {
+    const bool in_key = (t >= kw0 && t < kw1);
+    const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, t);
     V3<float> gr{0.0f, 0.0f, 0.0f};
     const float alpha = fmaf(cs, VG ? trilinear_vg(vol, g, q.x, q.y, q.z, gr) : trilinear(vol, g, q.x, q.y, q.z), ci);
-    const float t = (float)(kst - k);
+    const float ts = t;
     if (SKIP) {
       if (alpha < 0.0f && !in_key) {
         sa = sb;
         sb = 0;
-        if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; k += 1; }
-        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; k -= 1; }
-        if (!sa) { add(alpha); visit(q, t, alpha, gr); }
+        if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; t -= 1.0f; }
+        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; t += 1.0f; }
+        if (!sa) { acc += (Acc)alpha; visit(q, ts, alpha, gr); }
         if (acc <= stop_at) break;
       } else {
-        if (!sa) { step(p, inc); k--; }
+        if (!sa) { step(p, inc); t += 1.0f; }
         sb = !sa;
         step(p, inc);
-        k--;
+        t += 1.0f;
         sa = 0;
       }
     } else {
       step(p, inc);
-      k--;
+      t += 1.0f;
       if (alpha < 0.0f && !in_key) {
-        add(alpha);
-        visit(q, t, alpha, gr);
+        acc += (Acc)alpha;
+        visit(q, ts, alpha, gr);
         if (acc <= stop_at) break;
       }
     }
   }
+  const float L = (float)acc;
   if (ref_mode) {
-    const float o = 256.0f - expf(acc);
+    const float o = 256.0f - expf(L);
     return o < 0.0f ? 0.0f : o;
   }
-  if (a.output == RVK_DRR_F32_EXP_NEG_L) return expf(acc);
-  return -acc;
+  if (a.output == RVK_DRR_F32_EXP_NEG_L) return expf(L);
+  return -L;
 }
 
 }  // namespace rvk

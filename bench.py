@@ -8,8 +8,9 @@ Prints ONE JSON line.  `value` = N_samples / (seconds per render) / 1e9, where
 N_samples is the plain-march ray-sample count (sum over pixels of
 floor(num_steps - ray offset)).  `roofline` prices the caster kernel's
 gather-equivalent bytes (8 voxels x 2 B per sample + 4 B per pixel) against
-the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (FP32, all granted
-host threads) on the same frame.
+the 8 TB/s HBM peak, and `roofline.traffic` is the kernel's real HBM bytes per
+launch from the committed rocprofv3 counter passes; `cpu_baseline` times the
+CPU oracle (FP32, all granted host threads) on the same frame.
 """
 from __future__ import annotations
 
@@ -29,6 +30,18 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter passes
+    (profiles/drr_kernel_c2_traffic.json: FETCH_SIZE and WRITE_SIZE scaled by
+    a 256 MiB copy measured in the same runs), or None."""
+    f = ROOT / "profiles" / "drr_kernel_c2_traffic.json"
+    try:
+        rec = json.loads(f.read_text())
+    except (OSError, ValueError):
+        return None
+    return int(rec["bytes_per_launch"]) if rec.get("kernel") == kernel else None
 
 
 def cpu_threads() -> int:
@@ -281,7 +294,8 @@ def main() -> int:
                                "1024x1024 float32 exp(-L), step 0.5 voxel, skip rule on, precise march",
                    "n_samples": r["n_samples"], "voxel_bytes": 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("drr_kernel<short,true,true>"),
                      "kernel": "drr_kernel<short,true,true>", "kernel_ms": round(r["kernel_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes},
     }

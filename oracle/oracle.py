@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
             ("orc_dual_f32", [P, P, P, P, P, P, P, P, C.c_int], C.c_int),
             ("orc_dual_f64", [P, P, P, P, P, P, P, P, C.c_int], C.c_int),
             ("orc_count_samples", [P, P, P, P, P, C.c_int], C.c_longlong),
+            ("orc_unsafe_rays", [C.c_int], C.c_longlong),
             ("orc_sample_f64", [P, C.c_double, C.c_double, C.c_double], C.c_double),
             ("orc_sample_f32", [P, C.c_float, C.c_float, C.c_float], C.c_float),
             ("orc_tf1d_lookup_f32", [P, C.c_int, C.c_float], C.c_float),
@@ -389,3 +390,9 @@ def tf1d_lookup(table: np.ndarray, u: float) -> float:
 def tf2d_lookup(table: np.ndarray, u: float, v: float) -> float:
     t = np.ascontiguousarray(table, np.float32)
     return float(lib().orc_tf2d_lookup_f32(t.ctypes.data, t.shape[0], float(u), float(v)))
+
+
+def unsafe_rays(reset: bool = True) -> int:
+    """Rays the precise DRR march took no samples of because their sample
+    range left the cell range [0, n + 1] (expected 0 for volume-clipped rays)."""
+    return int(lib().orc_unsafe_rays(int(reset)))

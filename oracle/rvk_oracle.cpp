@@ -155,6 +155,28 @@ template <typename T, typename VT> inline T trilinear(const Grid<VT>& g, T cx, T
   return lerp(c0, c1, az);
 }
 
+// The precise DRR march samples at c = (p0 + 0.5) + t inc, t in [0, k - 1]
+// (one fma per axis, monotone in t).  The GPU DRR tests c at t = 0 and t = k
+// against the cell range [0, n + 1] per axis and then drops the per-sample
+// clamp; a ray that fails takes no samples.  Rays that pass the volume clip
+// sample within [0.5, n - 0.5] and always pass (tests/test_oracle.py counts
+// the rays that fail on every scene).
+static long long g_unsafe_rays = 0;
+template <typename T, typename VT>
+bool samples_in_cells(const Grid<VT>& g, const V3<T>& p, const V3<T>& inc, int k) {
+  const T t = (T)k;
+  const T s[3] = {p.x + T(0.5), p.y + T(0.5), p.z + T(0.5)};
+  const T d[3] = {inc.x, inc.y, inc.z};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  bool ok = true;
+  for (int a = 0; a < 3; ++a) {
+    const T e = std::fma(t, d[a], s[a]);
+    ok = ok && s[a] >= T(0) && s[a] <= (T)(n[a] + 1) && e >= T(0) && e <= (T)(n[a] + 1);
+  }
+  if (!ok && k > 0) __atomic_add_fetch(&g_unsafe_rays, 1, __ATOMIC_RELAXED);
+  return ok;
+}
+
 // 1D TF: tex1D, normalised, linear, clamp (V/...1D...cuh:248-267).
 template <typename T> inline T tf1d(const float* t, int n, T u) {
   T x = u * (T)n - T(0.5);
@@ -400,6 +422,7 @@ T drr_ray(const Grid<VT>& g, const VolInfo<T>& vi, const orc_march* mp, const or
   const T j = jitter_of<T>(mp, x, y);
   int k = step_count(r.num_steps - j);
   V3<T> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  if (mp->precise && !samples_in_cells(g, p, r.inc, k)) k = 0;
   const V3<T>& inc = r.inc;
   const T ds = std::sqrt(inc.x * inc.x * vi.sp[0] * vi.sp[0] + inc.y * inc.y * vi.sp[1] * vi.sp[1] +
                          inc.z * inc.z * vi.sp[2] * vi.sp[2]);
@@ -1184,6 +1207,13 @@ int orc_post_shade_inex_f32(const orc_shade* sp, int W, int H, float* depth, con
       }
     }
   return 0;
+}
+
+// Rays the precise marches dropped since the last reset (see samples_in_cells).
+long long orc_unsafe_rays(int reset) {
+  const long long v = __atomic_load_n(&g_unsafe_rays, __ATOMIC_RELAXED);
+  if (reset) __atomic_store_n(&g_unsafe_rays, 0, __ATOMIC_RELAXED);
+  return v;
 }
 
 long long orc_count_samples(const orc_volume* vol, const orc_view* view, const orc_march* mp, const orc_tf1d* tf1d,

@@ -150,6 +150,9 @@ int orc_post_shade_f32(const orc_shade* sp, int W, int H, float* depth, const fl
  * (no skip rule) from the jittered clipped entry to the ray exit.  For the
  * compositing casters the march also stops at the transmittance threshold;
  * pass tf1d or tf2d (or neither, for the DRR). */
+/* Rays the precise DRR march took no samples of (their ends left the cell
+ * range [0, n + 1]) since the last reset; 0 for every volume-clipped ray. */
+long long orc_unsafe_rays(int reset);
 long long orc_count_samples(const orc_volume* vol, const orc_view* view, const orc_march* mp,
                             const orc_tf1d* tf1d, const orc_tf2d* tf2d, int nthreads);
 

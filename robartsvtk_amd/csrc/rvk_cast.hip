@@ -101,15 +101,15 @@ __global__ __launch_bounds__(256) void cast1d_kernel(Cast1dArgs a) {
   const bool use_galpha = isfinite(gmult);
   const float term = a.term_T;
   const float j = ray_offset(a.jitter, x, y);
-  int k = step_count(r.num_steps - j);
-  const int k0 = k;
   V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
   const V3<float> inc = r.inc;
+  const V3<float> p0c = cell_coord(p);
+  int k = step_count(r.num_steps - j);
+  const int k0 = k;
   const float rlen = ray_length(g, inc);
   const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
   const int es = (int)ceilf(r.exclude_start), ee = (int)floorf(r.exclude_end);
   const int kst = k;
-  const V3<float> p0c = cell_coord(p);
   int sa = 0, sb = 0;
 
   auto accumulate = [&](float alpha, float u) {
@@ -235,10 +235,11 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
   const float glo = a.p.gradient_low, gmult = a.p.gradient_multiplier, goff = a.p.gradient_offset;
   const float term = a.term_T;
   const float j = ray_offset(a.jitter, x, y);
-  int k = step_count(r.num_steps - j);
-  const int k0 = k;
   V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
   const V3<float> inc = r.inc;
+  const V3<float> p0c = cell_coord(p);
+  int k = step_count(r.num_steps - j);
+  const int k0 = k;
   const float rlen = ray_length(g, inc);
   const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
   // WithDual rounds the exclusion bounds up/down, WithoutDual down/up
@@ -247,7 +248,6 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
   es = k - es;
   ee = k - ee;
   const int kst = k;
-  const V3<float> p0c = cell_coord(p);
   int sa = 0, sb = 0;
   while (k > 0) {
     const bool in_key = (es >= k && ee < k);

@@ -68,6 +68,7 @@ __global__ __launch_bounds__(256) void inex_kernel(InExArgs a) {
   const float term = a.term_T;
   const float j = ray_offset(a.jitter, x, y);
   V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  const V3<float> p0c = cell_coord(p);
   int k = step_count(r.num_steps);
   const float depth0 = j + (float)k;
   const int es = k - (int)ceilf(r.exclude_start);
@@ -76,7 +77,6 @@ __global__ __launch_bounds__(256) void inex_kernel(InExArgs a) {
   const float rlen = ray_length(g, inc);
   const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
   const int kst = k;
-  const V3<float> p0c = cell_coord(p);
   bool skip = false, back = false, special = false;
   while (k > 0) {
     const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
@@ -208,6 +208,7 @@ __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
   const float term = a.term_T;
   const float j = ray_offset(a.jitter, x, y);
   V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
+  const V3<float> p0c = cell_coord(p);
   int k = dual_step_count(r.num_steps, j);
   const int k0 = k;
   const int es = k - (int)ceilf(r.exclude_start);
@@ -216,7 +217,6 @@ __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
   const float rlen = ray_length(g, inc);
   const float sx = g.sp[0], sy = g.sp[1], sz = g.sp[2];
   const int kst = k;
-  const V3<float> p0c = cell_coord(p);
   int sa = 0, sb = 0;
   while (k > 0) {
     const bool in_key = (es >= k && ee < k);

@@ -218,11 +218,12 @@ struct CellPos {
   float ax, ay, az;
   bool xs, ys, zs;  // the two corners along the axis are distinct voxels
 };
-template <int L>
+// CLAMP = false when the caller has checked that c lies in [0, n + 1].
+template <int L, bool CLAMP = true>
 __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float cx, float cy, float cz) {
-  const float x = __builtin_amdgcn_fmed3f(cx, 0.0f, (float)(g.n[0] + 1));
-  const float y = __builtin_amdgcn_fmed3f(cy, 0.0f, (float)(g.n[1] + 1));
-  const float z = __builtin_amdgcn_fmed3f(cz, 0.0f, (float)(g.n[2] + 1));
+  const float x = CLAMP ? __builtin_amdgcn_fmed3f(cx, 0.0f, (float)(g.n[0] + 1)) : cx;
+  const float y = CLAMP ? __builtin_amdgcn_fmed3f(cy, 0.0f, (float)(g.n[1] + 1)) : cy;
+  const float z = CLAMP ? __builtin_amdgcn_fmed3f(cz, 0.0f, (float)(g.n[2] + 1)) : cz;
   CellPos c;
   c.ax = __builtin_amdgcn_fractf(x);
   c.ay = __builtin_amdgcn_fractf(y);
@@ -238,10 +239,10 @@ __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float cx, float cy
 // a + t (b - a) as one rounding of the product-sum
 __device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
 
-// Trilinear value at the cell coordinate c = p + 0.5.
-template <typename T, int L>
+// Trilinear value at the cell coordinate c = p + 0.5 (CLAMP as for cell_pos).
+template <bool CLAMP = true, typename T, int L>
 __device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, float cx, float cy, float cz) {
-  const CellPos c = cell_pos<L>(g, cx, cy, cz);
+  const CellPos c = cell_pos<L, CLAMP>(g, cx, cy, cz);
   float v[8];
   CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
@@ -257,10 +258,10 @@ __device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, fl
 // d/dx = lerp_z(lerp_y(v100-v000, v110-v010), lerp_y(v101-v001, v111-v011)),
 // d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0; axes whose two corners
 // clamp to one voxel get a zero derivative.
-template <typename T, int L>
+template <bool CLAMP = true, typename T, int L>
 __device__ __forceinline__ float trilinear_vg(Cells<T, L> vol, const VolGeom& g, float cx, float cy, float cz,
                                               V3<float>& gr) {
-  const CellPos c = cell_pos<L>(g, cx, cy, cz);
+  const CellPos c = cell_pos<L, CLAMP>(g, cx, cy, cz);
   float v[8];
   CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
   const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);

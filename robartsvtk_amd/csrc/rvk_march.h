@@ -78,6 +78,29 @@ __device__ __forceinline__ V3<float> cell_coord(const V3<float>& p) {
   return V3<float>{p.x + 0.5f, p.y + 0.5f, p.z + 0.5f};
 }
 
+// The precise DRR march samples at c = p0c + t inc for t in [0, kst - 1], one
+// fma per axis, which is monotone in t.  So when c at t = 0 and at t = kst
+// lies in the cell range [0, n + 1] on every axis, every sample does and
+// trilinear() needs no clamp.  Rays that pass the volume clip have both ends
+// within [0, n - 1] in voxels (V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:297-311),
+// so their samples lie in [0.5, n - 0.5] and always pass; a ray that fails
+// (a non-finite set-up) takes no samples, in the oracle as here.  (The
+// compositing casters keep the clamp: their +-1 gradient taps reach
+// [-0.5, n + 0.5].)
+__device__ __forceinline__ bool samples_in_cells(const VolGeom& g, const V3<float>& p0c, const V3<float>& inc,
+                                                 int kst) {
+  const float t = (float)kst;
+  const float s[3] = {p0c.x, p0c.y, p0c.z};
+  const float e[3] = {fmaf(t, inc.x, p0c.x), fmaf(t, inc.y, p0c.y), fmaf(t, inc.z, p0c.z)};
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float hi = (float)(g.n[a] + 1);
+    ok = ok && s[a] >= 0.0f && s[a] <= hi && e[a] >= 0.0f && e[a] <= hi;
+  }
+  return ok;
+}
+
 // ===========================================================================
 // DRR march (V/CUDA_vtkCudaDRRImageVolumeMapper_renderAlgo.cuh:11-117)
 struct DrrMarchArgs {
@@ -108,9 +131,11 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
   // termination at acc <= stop_at only with ref_term (otherwise never)
   const Acc stop_at = !a.ref_term ? (Acc)-__builtin_huge_valf() : (ref_mode ? (Acc)0 : (Acc)-lg);
   const float j = ray_offset(a.jitter, x, y);
-  const int kst = step_count(r.num_steps - j);
   V3<float> p{r.start.x + j * r.inc.x, r.start.y + j * r.inc.y, r.start.z + j * r.inc.z};
   const V3<float> inc = r.inc;
+  const V3<float> p0c = cell_coord(p);
+  int kst = step_count(r.num_steps - j);
+  if (PRECISE && !samples_in_cells(g, p0c, inc, kst)) kst = 0;
   const float ds = ray_length(g, inc);
   const float ci = a.intercept * -ds;
   const float cs = a.slope * -ds;
@@ -120,14 +145,14 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
   const int es = kst - (int)ceilf(r.exclude_start);
   const int ee = kst - (int)floorf(r.exclude_end);
   const float tend = (float)kst, kw0 = (float)(kst - es), kw1 = (float)(kst - ee);
-  const V3<float> p0c = cell_coord(p);
   float t = 0.0f;
   int sa = 0, sb = 0;
   while (t < tend) {
     const bool in_key = (t >= kw0 && t < kw1);
     const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, t);
     V3<float> gr{0.0f, 0.0f, 0.0f};
-    const float alpha = fmaf(cs, VG ? trilinear_vg(vol, g, q.x, q.y, q.z, gr) : trilinear(vol, g, q.x, q.y, q.z), ci);
+    const float alpha = fmaf(cs, VG ? trilinear_vg<!PRECISE>(vol, g, q.x, q.y, q.z, gr)
+                                    : trilinear<!PRECISE>(vol, g, q.x, q.y, q.z), ci);
     const float ts = t;
     if (SKIP) {
       if (alpha < 0.0f && !in_key) {

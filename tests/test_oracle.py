@@ -592,3 +592,49 @@ def test_post_shade_identity_at_zero_darkness():
     shade = {"celr": 0.0, "cela": 0.002, "celc": 1 / 0.003, "disr": 0.0, "disa": 0.45, "disc": 1 / 0.15}
     _, out = orc.post_shade(shade, depth, mind, maxd, rgba)
     np.testing.assert_array_equal(out, rgba)
+
+
+def test_precise_marches_stay_inside_the_cell_range():
+    """The precise DRR march skips the per-sample clamp after checking that a
+    ray's sample range (t in [0, k], cell coordinates) lies in the cell range
+    [0, n + 1]; a ray failing the check takes no samples.  Rays that pass the
+    volume clip never fail: counted over cameras outside, grazing, inside and
+    behind the volume, parallel and perspective, flipped, with user clipping
+    and keyhole planes (the other casters run too and must not count)."""
+    s1 = configs.scene_c1(scale=0.25)
+    s3 = configs.scene_c3(scale=0.125)
+    dims, sp = s1.dims, s1.spacing
+    c = configs.volume_center(dims, sp)
+    ext = np.array(dims) * np.array(sp)
+    cams = [s1.extra["camera"],
+            abi.camera(c, c + np.array([1.0, 0.3, 0.2]), (0, 0, 1), 60.0, (0.1, 500.0)),          # inside
+            abi.camera(c + np.array([0, 0, 3 * ext[2]]), c, (0, 1, 0), 20.0, (1.0, 500.0),      # parallel
+                       parallel=True, parallel_scale=0.6 * ext[0]),
+            abi.camera(c + np.array([ext[0] * 0.5 + 0.01, 0, 2 * ext[2]]), c + np.array([ext[0] * 0.5 + 0.01, 0, 0]),
+                       (0, 1, 0), 25.0, (1.0, 500.0)),                                          # grazing a face
+            abi.camera(c + np.array([0, 0, -2 * ext[2]]), c + np.array([0, 0, -4 * ext[2]]), (0, 1, 0), 30.0,
+                       (1.0, 500.0))]                                                            # looking away
+    orc.unsafe_rays(True)
+    for cam in cams:
+        for flip in (0, 1):
+            view = abi.compute_view(cam, 48, 40, spacing=sp)
+            view.flipped = flip
+            vd = configs.view_dict(view)
+            march = dict(s1.march, precise=1)
+            orc.cast1d(s1.volume, sp, s1.tf1d, vd, march, "f32")
+            orc.drr(s1.volume.astype(np.int16), sp, vd, march,
+                    {"intercept": 0.0, "slope": 1e-3, "offset": 1.0, "output": 1, "ref_termination": 0}, "f32")
+            key = dict(vd, keyhole=[(1, 0, 0, -0.3 * dims[0]), (-1, 0, 0, 0.7 * dims[0]), (0, 1, 0, -0.3 * dims[1]),
+                                    (0, -1, 0, 0.7 * dims[1]), (0, 0, 1, -0.2 * dims[2]), (0, 0, -1, 0.8 * dims[2])],
+                       clip=[(1, 0.1, 0, -0.2 * dims[0]), (-1, 0, 0.1, 0.9 * dims[0]), (0, 1, 0, 0.0),
+                             (0, -1, 0, float(dims[1])), (0, 0, 1, 0.0), (0, 0, -1, float(dims[2]))])
+            orc.cast1d(s1.volume, sp, s1.tf1d, key, march, "f32")
+            orc.drr(s1.volume.astype(np.int16), sp, key, march,
+                    {"intercept": 0.0, "slope": 1e-3, "offset": 1.0, "output": 1, "ref_termination": 0}, "f32")
+            vol3 = s3.volume[:dims[2], :dims[1], :dims[0]]
+            orc.cast2d(vol3, sp, s3.tf2d, key, dict(s3.march, precise=1), "f32")
+            N = s3.tf2d["tables"]["alpha"].shape[0]
+            orc.cast2d_inex(vol3, sp, s3.tf2d, np.full((N, N), 0.3, np.float32), True, key,
+                            dict(s3.march, precise=1), "f32")
+            orc.cast_dual(vol3, vol3, sp, s3.tf2d, key, dict(s3.march, precise=1), "f32")
+    assert orc.unsafe_rays(True) == 0

@@ -174,8 +174,12 @@ int rvk_copy_to_host(rvk_ctx* ctx, void* dst, const void* src, size_t bytes);
  * Replaces CUDA_castBuffer<T,float> + *_loadImageInfo / changeFrame /
  * clearImageArray (V/CUDA_vtkCudaVolumeMapper_renderAlgo.h:30-35,
  * V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.h:74-77).  Frames keep their native
- * type in a 4x4x4-bricked device layout; src is x-fastest (VTK order) host
- * or device memory, borrowed only for the duration of the call. */
+ * type in a device cell layout (see rvk_volume_set_layout); src is x-fastest
+ * (VTK order) host or device memory, borrowed only for the duration of the
+ * call.  Float cells store each x-pair of voxels as (a, b - a); so do 16-bit
+ * cells when the frame's values span less than 32768 (checked on the device
+ * at upload, which then waits for the check).  Renders are bit-identical in
+ * either encoding. */
 int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[3], int dtype,
                       int n_frames, rvk_volume** out);
 int rvk_volume_set_frame(rvk_volume* vol, int frame, const void* src, int src_on_device);
@@ -298,7 +302,10 @@ int rvk_drr_plan_autotune(rvk_drr_plan* plan, float* out, int reps, float* best_
 int rvk_drr_plan_render(rvk_drr_plan* plan, float* out);
 /* Capture render(out) (and, with targets, the registration step below) into
  * a hipGraph; later rvk_drr_plan_render / rvk_drr_plan_register calls with
- * the same buffers replay it.  Passing NULL drops the graph. */
+ * the same buffers replay it (as long as the frame keeps the cell encoding
+ * it had at capture; a re-uploaded 16-bit frame whose value range crossed
+ * 32767 is rendered by a direct launch instead).  Passing NULL drops the
+ * graph. */
 int rvk_drr_plan_capture(rvk_drr_plan* plan, float* out, const float* targets, float* loss, float* grad_pose);
 
 /* Zero-mean normalised cross-correlation of every view with packed target

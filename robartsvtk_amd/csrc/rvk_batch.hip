@@ -421,7 +421,7 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
   const int i = (int)(vid % g.n[0]);
   const int jj = (int)((vid / g.n[0]) % g.n[1]);
   const int k = (int)(vid / ((long long)g.n[0] * g.n[1]));
-  const Cells<float, L> vol = cells_of<Cells<float, L>>(a.b.vol);
+  const Cells<float, L, true> vol = cells_of<Cells<float, L, true>>(a.b.vol);
   // With mu = value (slope > 0, intercept 0) a sample is accumulated exactly
   // when its trilinear value is > 0.  Samples in this voxel's support read
   // cells inside [i-1, i+1]^3, so if every value there is <= 0 none is.
@@ -538,6 +538,7 @@ struct rvk_drr_plan {
   const float* g_tgt = nullptr;
   float* g_loss = nullptr;
   float* g_grad = nullptr;
+  int g_diff = 0;  // cell encoding of the frame when the graph was captured
 };
 
 namespace {
@@ -610,7 +611,7 @@ BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
 int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s, bool jac = false) {
   BatchArgs a = batch_args(p, out);
   const bool skip = p->march.skip_rule != 0, pr = p->march.precise != 0;
-  return dispatch_cells(p->vol->dtype, p->vol->layout, [&](auto tag) {
+  return dispatch_frame(p->vol, p->frame, [&](auto tag) {
     using CV = decltype(tag);
     const int sched = jac ? p->jac_schedule : p->schedule;
     const int wps = jac ? p->jac_waves_per_simd : p->waves_per_simd;
@@ -858,7 +859,7 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
 int rvk_drr_plan_render(rvk_drr_plan* p, float* out) {
   if (!p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && p->g_out == out && !p->g_tgt) {
+  if (p->exec && p->g_out == out && !p->g_tgt && p->g_diff == p->vol->diff[p->frame]) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }
@@ -888,6 +889,7 @@ int rvk_drr_plan_capture(rvk_drr_plan* p, float* out, const float* tgt, float* l
   p->g_tgt = tgt;
   p->g_loss = loss;
   p->g_grad = grad;
+  p->g_diff = p->vol->diff[p->frame];
   return RVK_OK;
 }
 
@@ -895,7 +897,8 @@ int rvk_drr_plan_register(rvk_drr_plan* p, float* out, const float* tgt, float* 
   if (!p || !out || !tgt || !loss || !grad) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad) {
+  if (p->exec && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad &&
+      p->g_diff == p->vol->diff[p->frame]) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }
@@ -929,7 +932,7 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     aa.stats = nullptr;
     aa.tgt = nullptr;
     aa.part = p->d_pose_part;
-    int rc = dispatch_cells(p->vol->dtype, p->vol->layout, [&](auto tag) {
+    int rc = dispatch_frame(p->vol, p->frame, [&](auto tag) {
       using CV = decltype(tag);
       if (p->march.skip_rule) hipLaunchKernelGGL((adjoint_pose_kernel<CV, true>), dim3(blocks), dim3(256), 0, s, aa);
       else hipLaunchKernelGGL((adjoint_pose_kernel<CV, false>), dim3(blocks), dim3(256), 0, s, aa);

@@ -419,7 +419,7 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
   a.vol = vol->frames[frame];
   a.out = out;
   const int nblk = launch_grid(view, a.tiles_x);
-  return dispatch_cells(vol->dtype, vol->layout, [&](auto tag) {
+  return dispatch_frame(vol, frame, [&](auto tag) {
     using CV = decltype(tag);
     const bool pr = march->precise != 0, skip = march->skip_rule != 0;
     if (skip && pr) hipLaunchKernelGGL((drr_kernel<CV, true, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);
@@ -458,7 +458,7 @@ int rvk_render_1d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf1d
   const bool grad = !(tf->p.diffuse == 0.0f && tf->p.specular == 0.0f && tf->p.specular_power >= 0.0f &&
                       !std::isfinite(tf->p.gradient_multiplier));
   const bool skip = march->skip_rule != 0, pr = march->precise != 0;
-  return dispatch_cells(vol->dtype, vol->layout, [&](auto tag) {
+  return dispatch_frame(vol, frame, [&](auto tag) {
     using CV = decltype(tag);
 #define RVK_L1D(S, G, P) \
   hipLaunchKernelGGL((cast1d_kernel<CV, S, G, P>), dim3(nblk), dim3(256), lds, ctx->stream, a)
@@ -511,7 +511,7 @@ int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d
   const int nblk = launch_grid(view, a.tiles_x);
   const bool skip = march->skip_rule != 0, dual = tf->kcells != nullptr, gm = use_gradmag != 0;
   const bool pr = march->precise != 0;
-  return dispatch_cells(vol->dtype, vol->layout, [&](auto tag) {
+  return dispatch_frame(vol, frame, [&](auto tag) {
     using CV = decltype(tag);
 #define RVK_L2D(S, D, G)                                                                              \
   do {                                                                                                \

@@ -10,6 +10,7 @@
 //   In/Ex  V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.cuh:24-179, 241-302
 //   dual   V/CUDA_vtkCudaDualImageVolumeMapper_renderAlgo.cuh:33-380, 383-517
 #include <cmath>
+#include <type_traits>
 
 #include "rvk_internal.h"
 #include "rvk_march.h"
@@ -181,7 +182,7 @@ __device__ __forceinline__ int dual_step_count(float num_steps, float j) {
 // (component 1 the column, component 2 the row); shading uses the gradient of
 // component 1.  KEYTF selects the keyhole-TF variant; without it the march
 // jumps over the keyhole window.  Both round the window bounds up/down.
-template <typename CV, bool SKIP, bool KEYTF, bool PRECISE>
+template <typename CV, typename CV2, bool SKIP, bool KEYTF, bool PRECISE>
 __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
   extern __shared__ uint32_t s_occ[];
   const bool use_occ = a.occ != nullptr;
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
   if (x >= a.v.W || y >= a.v.H) return;
   const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
   const CV vol1 = cells_of<CV>(a.vol1);
-  const CV vol2 = cells_of<CV>(a.vol2);
+  const CV2 vol2 = cells_of<CV2>(a.vol2);
   const VolGeom& g = a.g;
   const int n = a.n;
 
@@ -326,7 +327,7 @@ int rvk_render_2d_inex(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk
   a.max_depth = aux ? aux->max_depth : nullptr;
   const int nblk = launch_grid(view, a.tiles_x);
   const bool skip = march->skip_rule != 0, pr = march->precise != 0;
-  return dispatch_cells(vol->dtype, vol->layout, [&](auto tag) {
+  return dispatch_frame(vol, frame, [&](auto tag) {
     using CV = decltype(tag);
 #define RVK_LIE(S, P) hipLaunchKernelGGL((inex_kernel<CV, S, P>), dim3(nblk), dim3(256), lds, ctx->stream, a)
     if (skip) { if (pr) RVK_LIE(true, true); else RVK_LIE(true, false); }
@@ -373,18 +374,28 @@ int rvk_render_dual(rvk_ctx* ctx, const rvk_volume* vol1, const rvk_volume* vol2
   a.max_depth = aux ? aux->max_depth : nullptr;
   const int nblk = launch_grid(view, a.tiles_x);
   const bool skip = march->skip_rule != 0, key = tf->kcells != nullptr, pr = march->precise != 0;
-  return dispatch_cells(vol1->dtype, vol1->layout, [&](auto tag) {
-    using CV = decltype(tag);
-#define RVK_LDU(S, K)                                                                                           \
-  do {                                                                                                          \
-    if (pr) hipLaunchKernelGGL((dual_kernel<CV, S, K, true>), dim3(nblk), dim3(256), lds, ctx->stream, a);     \
-    else hipLaunchKernelGGL((dual_kernel<CV, S, K, false>), dim3(nblk), dim3(256), lds, ctx->stream, a);       \
+  // the two components share voxel type and cell order, but each frame has
+  // its own cell encoding
+  return dispatch_frame(vol1, frame, [&](auto tag1) {
+    using CV = decltype(tag1);
+    return dispatch_frame(vol2, frame, [&](auto tag2) {
+      using CV2 = decltype(tag2);
+      if constexpr (std::is_same<typename CV::value_type, typename CV2::value_type>::value &&
+                    CV::kLog2 == CV2::kLog2) {
+#define RVK_LDU(S, K)                                                                                               \
+  do {                                                                                                              \
+    if (pr) hipLaunchKernelGGL((dual_kernel<CV, CV2, S, K, true>), dim3(nblk), dim3(256), lds, ctx->stream, a);    \
+    else hipLaunchKernelGGL((dual_kernel<CV, CV2, S, K, false>), dim3(nblk), dim3(256), lds, ctx->stream, a);      \
   } while (0)
-    if (skip) { if (key) RVK_LDU(true, true); else RVK_LDU(true, false); }
-    else { if (key) RVK_LDU(false, true); else RVK_LDU(false, false); }
+        if (skip) { if (key) RVK_LDU(true, true); else RVK_LDU(true, false); }
+        else { if (key) RVK_LDU(false, true); else RVK_LDU(false, false); }
 #undef RVK_LDU
-    RVK_HIP(hipGetLastError());
-    return (int)RVK_OK;
+        RVK_HIP(hipGetLastError());
+        return (int)RVK_OK;
+      } else {
+        return fail(RVK_ERR_INVALID_ARGUMENT, "the two component volumes differ in type or cell layout");
+      }
+    });
   });
 }
 

@@ -111,17 +111,22 @@ constexpr int kCellLayoutPlain = 0;
 constexpr int kCellLayoutBrick4 = 2;
 
 // A cell volume of voxel type T in cell order LOG2 (what the kernels are
-// templated on; `d` is the device array of cells).
-template <typename T, int LOG2> struct Cells {
+// templated on; `d` is the device array of cells).  DIFF cells store each
+// x-pair as (a, b - a) instead of (a, b): the first lerp level then needs no
+// subtraction.  The difference is exact as stored (16-bit frames whose value
+// range is below 32768; f32 stores the rounded difference the lerp would
+// compute), so values are bit-identical to those of plain cells.
+template <typename T, int LOG2, bool DIFF = false> struct Cells {
   using value_type = T;
   using order = CellOrder<LOG2>;
   static constexpr int kLog2 = LOG2;
+  static constexpr bool kDiff = DIFF;
   const T* __restrict__ d;
 };
 
-// The 8 corner values of a cell as floats, in the order dx + 2 dy + 4 dz.
-// A cell's raw bits (`Raw`, what one vector load returns) and their 8 corner
-// values as floats (`unpack`), so a march can load a cell ahead of its use.
+// A cell's raw bits (`Raw`, what one vector load returns) and their 8 slots
+// as floats (`unpack`, in the order dx + 2 dy + 4 dz).  `unpack_ad` gives the
+// four x-pairs as a = v(dx=0) and d = v(dx=1) - v(dx=0) for either encoding.
 template <typename VT> struct CellLoad;
 template <> struct CellLoad<uint8_t> {
   using Raw = uint2;
@@ -133,6 +138,15 @@ template <> struct CellLoad<uint8_t> {
     for (int i = 0; i < 4; ++i) {
       v[i] = (float)((w.x >> (8 * i)) & 0xffu);
       v[4 + i] = (float)((w.y >> (8 * i)) & 0xffu);
+    }
+  }  template <bool DIFF> static __device__ __forceinline__ void unpack_ad(Raw w, float a[4], float d[4]) {
+    static_assert(!DIFF, "8-bit cells are not difference-encoded");
+    float v[8];
+    unpack(w, v);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = v[2 * i];
+      d[i] = v[2 * i + 1] - v[2 * i];
     }
   }
 };
@@ -146,6 +160,15 @@ template <> struct CellLoad<int8_t> {
     for (int i = 0; i < 4; ++i) {
       v[i] = (float)((int)((uint32_t)w.x << (24 - 8 * i)) >> 24);
       v[4 + i] = (float)((int)((uint32_t)w.y << (24 - 8 * i)) >> 24);
+    }
+  }  template <bool DIFF> static __device__ __forceinline__ void unpack_ad(Raw w, float a[4], float d[4]) {
+    static_assert(!DIFF, "8-bit cells are not difference-encoded");
+    float v[8];
+    unpack(w, v);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = v[2 * i];
+      d[i] = v[2 * i + 1] - v[2 * i];
     }
   }
 };
@@ -162,6 +185,15 @@ template <> struct CellLoad<uint16_t> {
       v[2 * i + 1] = (float)(q[i] >> 16);
     }
   }
+  template <bool DIFF> static __device__ __forceinline__ void unpack_ad(Raw w, float a[4], float d[4]) {
+    const uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = (float)(q[i] & 0xffffu);
+      // a DIFF slot holds the signed 16-bit difference
+      d[i] = DIFF ? (float)((int)q[i] >> 16) : (float)(q[i] >> 16) - a[i];
+    }
+  }
 };
 template <> struct CellLoad<int16_t> {
   using Raw = int4;
@@ -174,6 +206,14 @@ template <> struct CellLoad<int16_t> {
     for (int i = 0; i < 4; ++i) {
       v[2 * i] = (float)((int)((uint32_t)q[i] << 16) >> 16);
       v[2 * i + 1] = (float)(q[i] >> 16);
+    }
+  }
+  template <bool DIFF> static __device__ __forceinline__ void unpack_ad(Raw w, float a[4], float d[4]) {
+    const int q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = (float)((int)((uint32_t)q[i] << 16) >> 16);
+      d[i] = DIFF ? (float)(q[i] >> 16) : (float)(q[i] >> 16) - a[i];
     }
   }
 };
@@ -189,6 +229,15 @@ template <typename W, typename V4> struct CellLoad32 {
     v[0] = (float)w.a.x; v[1] = (float)w.a.y; v[2] = (float)w.a.z; v[3] = (float)w.a.w;
     v[4] = (float)w.b.x; v[5] = (float)w.b.y; v[6] = (float)w.b.z; v[7] = (float)w.b.w;
   }
+  template <bool DIFF> static __device__ __forceinline__ void unpack_ad(const Raw& w, float a[4], float d[4]) {
+    float v[8];
+    unpack(w, v);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = v[2 * i];
+      d[i] = DIFF ? v[2 * i + 1] : v[2 * i + 1] - v[2 * i];
+    }
+  }
 };
 template <> struct CellLoad<float> : CellLoad32<float, float4> {};
 template <> struct CellLoad<int32_t> : CellLoad32<int32_t, int4> {};
@@ -199,8 +248,8 @@ template <typename CV> __device__ __forceinline__ CV cells_of(const void* p) {
 }
 
 // Voxel (i,j,k) (in range) = corner 000 of cell (i+1, j+1, k+1).
-template <typename T, int L>
-__device__ __forceinline__ float load_voxel(Cells<T, L> v, const VolGeom& g, int i, int j, int k) {
+template <typename T, int L, bool D>
+__device__ __forceinline__ float load_voxel(Cells<T, L, D> v, const VolGeom& g, int i, int j, int k) {
   return (float)v.d[(size_t)CellOrder<L>::index(g, i + 1, j + 1, k + 1) * 8];
 }
 
@@ -240,13 +289,14 @@ __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float cx, float cy
 __device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
 
 // Trilinear value at the cell coordinate c = p + 0.5 (CLAMP as for cell_pos).
-template <bool CLAMP = true, typename T, int L>
-__device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, float cx, float cy, float cz) {
+template <bool CLAMP = true, typename T, int L, bool D>
+__device__ __forceinline__ float trilinear(Cells<T, L, D> vol, const VolGeom& g, float cx, float cy, float cz) {
   const CellPos c = cell_pos<L, CLAMP>(g, cx, cy, cz);
-  float v[8];
-  CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
-  const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
-  const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
+  float a[4], d[4];
+  CellLoad<T>::template unpack_ad<D>(CellLoad<T>::raw(vol.d, c.cell), a, d);
+  // lerp(a, b, t) = fma(t, b - a, a) with b - a = d
+  const float c00 = fmaf(c.ax, d[0], a[0]), c10 = fmaf(c.ax, d[1], a[1]);
+  const float c01 = fmaf(c.ax, d[2], a[2]), c11 = fmaf(c.ax, d[3], a[3]);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
   return lerp(c0, c1, c.az);
 }
@@ -258,16 +308,16 @@ __device__ __forceinline__ float trilinear(Cells<T, L> vol, const VolGeom& g, fl
 // d/dx = lerp_z(lerp_y(v100-v000, v110-v010), lerp_y(v101-v001, v111-v011)),
 // d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0; axes whose two corners
 // clamp to one voxel get a zero derivative.
-template <bool CLAMP = true, typename T, int L>
-__device__ __forceinline__ float trilinear_vg(Cells<T, L> vol, const VolGeom& g, float cx, float cy, float cz,
+template <bool CLAMP = true, typename T, int L, bool D>
+__device__ __forceinline__ float trilinear_vg(Cells<T, L, D> vol, const VolGeom& g, float cx, float cy, float cz,
                                               V3<float>& gr) {
   const CellPos c = cell_pos<L, CLAMP>(g, cx, cy, cz);
-  float v[8];
-  CellLoad<T>::unpack(CellLoad<T>::raw(vol.d, c.cell), v);
-  const float c00 = lerp(v[0], v[1], c.ax), c10 = lerp(v[2], v[3], c.ax);
-  const float c01 = lerp(v[4], v[5], c.ax), c11 = lerp(v[6], v[7], c.ax);
+  float a[4], d[4];
+  CellLoad<T>::template unpack_ad<D>(CellLoad<T>::raw(vol.d, c.cell), a, d);
+  const float c00 = fmaf(c.ax, d[0], a[0]), c10 = fmaf(c.ax, d[1], a[1]);
+  const float c01 = fmaf(c.ax, d[2], a[2]), c11 = fmaf(c.ax, d[3], a[3]);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
-  const float dx0 = lerp(v[1] - v[0], v[3] - v[2], c.ay), dx1 = lerp(v[5] - v[4], v[7] - v[6], c.ay);
+  const float dx0 = lerp(d[0], d[1], c.ay), dx1 = lerp(d[2], d[3], c.ay);
   gr.x = c.xs ? lerp(dx0, dx1, c.az) : 0.0f;
   gr.y = c.ys ? lerp(c10 - c00, c11 - c01, c.az) : 0.0f;
   gr.z = c.zs ? c1 - c0 : 0.0f;

@@ -27,6 +27,7 @@ struct rvk_volume {
   int ncb[3] = {0, 0, 0};        // cell-brick grid: ceil((dims+2)/brick edge)
   size_t n_cells = 0;            // cells incl. brick padding (each 8 voxels)
   void* frames[RVK_MAX_FRAMES] = {};    // cell volumes, n_cells * 8 voxels
+  uint8_t diff[RVK_MAX_FRAMES] = {};    // 1: cells store x-pair differences (see Cells)
   uint8_t* gradmag[RVK_MAX_FRAMES] = {};  // u8 cell volumes
   float gradmag_scale[RVK_MAX_FRAMES] = {};
 };
@@ -81,22 +82,28 @@ int hip_fail(hipError_t e, const char* what);
 
 size_t dtype_size(int dtype);
 
-// Calls f(Cells<T, L>{nullptr}) for the volume's voxel type and cell order.
-template <int L, typename F> int dispatch_cells_t(int dtype, F&& f) {
+// Calls f(Cells<T, L, D>{nullptr}) for a frame's voxel type, cell order and
+// encoding: float cells always store x-pair differences, 16-bit cells when
+// the frame's value range allows (`diff`), 8- and 32-bit integer cells never.
+template <int L, typename F> int dispatch_cells_t(int dtype, bool diff, F&& f) {
   switch (dtype) {
     case RVK_U8: return f(Cells<uint8_t, L>{nullptr});
     case RVK_I8: return f(Cells<int8_t, L>{nullptr});
-    case RVK_U16: return f(Cells<uint16_t, L>{nullptr});
-    case RVK_I16: return f(Cells<int16_t, L>{nullptr});
+    case RVK_U16: return diff ? f(Cells<uint16_t, L, true>{nullptr}) : f(Cells<uint16_t, L>{nullptr});
+    case RVK_I16: return diff ? f(Cells<int16_t, L, true>{nullptr}) : f(Cells<int16_t, L>{nullptr});
     case RVK_U32: return f(Cells<uint32_t, L>{nullptr});
     case RVK_I32: return f(Cells<int32_t, L>{nullptr});
-    case RVK_F32: return f(Cells<float, L>{nullptr});
+    case RVK_F32: return f(Cells<float, L, true>{nullptr});
     default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
   }
 }
-template <typename F> int dispatch_cells(int dtype, int layout, F&& f) {
-  if (layout == kCellLayoutBrick4) return dispatch_cells_t<kCellLayoutBrick4>(dtype, f);
-  return dispatch_cells_t<kCellLayoutPlain>(dtype, f);
+template <typename F> int dispatch_cells(int dtype, int layout, bool diff, F&& f) {
+  if (layout == kCellLayoutBrick4) return dispatch_cells_t<kCellLayoutBrick4>(dtype, diff, f);
+  return dispatch_cells_t<kCellLayoutPlain>(dtype, diff, f);
+}
+// The cells of one frame of a volume.
+template <typename F> int dispatch_frame(const rvk_volume* v, int frame, F&& f) {
+  return dispatch_cells(v->dtype, v->layout, v->diff[frame] != 0, f);
 }
 VolGeom vol_geom(const rvk_volume* v);
 ViewGeom view_geom(const rvk_view* v);

@@ -155,26 +155,35 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
                                     : trilinear<!PRECISE>(vol, g, q.x, q.y, q.z), ci);
     const float ts = t;
     if (SKIP) {
+      // t stays an exact integer, so one add of +-1 or 2 equals the
+      // reference's single steps; a back-stepped sample adds +0 (acc is
+      // never -0), which leaves the sum unchanged without a select
       if (alpha < 0.0f && !in_key) {
         sa = sb;
         sb = 0;
-        if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; t -= 1.0f; }
-        else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; t += 1.0f; }
-        if (!sa) { acc += (Acc)alpha; visit(q, ts, alpha, gr); }
+        if (!PRECISE) {
+          if (sa) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; }
+          else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; }
+        }
+        t += sa ? -1.0f : 1.0f;
+        if (VG && !sa) visit(q, ts, alpha, gr);
+        acc += (Acc)(sa ? 0.0f : alpha);
         if (acc <= stop_at) break;
       } else {
-        if (!sa) { step(p, inc); t += 1.0f; }
+        if (!PRECISE) {
+          if (!sa) step(p, inc);
+          step(p, inc);
+        }
+        t += sa ? 1.0f : 2.0f;
         sb = !sa;
-        step(p, inc);
-        t += 1.0f;
         sa = 0;
       }
     } else {
-      step(p, inc);
+      if (!PRECISE) step(p, inc);
       t += 1.0f;
       if (alpha < 0.0f && !in_key) {
         acc += (Acc)alpha;
-        visit(q, ts, alpha, gr);
+        if (VG) visit(q, ts, alpha, gr);
         if (acc <= stop_at) break;
       }
     }

@@ -10,6 +10,7 @@
 //     hold each trilinear 2x2x2 footprint whole (8x the voxel bytes);
 //   the TF loadTextures functions (V/CUDA_vtkCuda1DVolumeMapper_renderAlgo.cuh:324-372,
 //     V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:589-651).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -126,7 +127,10 @@ template <> struct CellVec<int8_t> { using T = uint2; };
 template <> struct CellVec<uint16_t> { using T = uint4; };
 template <> struct CellVec<int16_t> { using T = uint4; };
 
-template <typename VT, int L>
+// DIFF stores each x-pair (a, b) as (a, b - a): exactly, as a 16-bit two's
+// complement difference (the frame's value range is below 32768) or, for
+// float, as the rounded difference the trilinear lerp computes from (a, b).
+template <typename VT, int L, bool DIFF = false>
 __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst, int nx,
                                                             int ny, int nz, int ncbx, int ncby, long long n_cells) {
   using O = CellOrder<L>;
@@ -148,6 +152,13 @@ __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict
     const int j = min(max(cy - 1 + ((q >> 1) & 1), 0), ny - 1);
     const int k = min(max(cz - 1 + (q >> 2), 0), nz - 1);
     v[q] = src[(long long)i + (long long)j * nx + (long long)k * nx * ny];
+  }
+  if (DIFF) {
+#pragma unroll
+    for (int q = 1; q < 8; q += 2) {
+      if constexpr (sizeof(VT) == 2) v[q] = (VT)(uint16_t)((int)v[q] - (int)v[q - 1]);
+      else v[q] = v[q] - v[q - 1];
+    }
   }
   VT* out = dst + c * 8;
   if constexpr (sizeof(VT) == 4) {
@@ -183,13 +194,57 @@ __global__ __launch_bounds__(256) void gradmag_kernel(CV vol, uint8_t* __restric
   out[e] = (uint8_t)fminf(rintf(q), 255.0f);
 }
 
-template <typename VT, int L>
+template <typename VT, int L, bool DIFF = false>
 static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
   const long long total = (long long)v->n_cells;
   const int blocks = (int)((total + 255) / 256);
-  hipLaunchKernelGGL((cell_relayout_kernel<VT, L>), dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc,
-                     (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
+  hipLaunchKernelGGL((cell_relayout_kernel<VT, L, DIFF>), dim3(blocks), dim3(256), 0, v->ctx->stream,
+                     (const VT*)dsrc, (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
   RVK_HIP(hipGetLastError());
+  return RVK_OK;
+}
+
+// Value range of a 16-bit frame (decides its cell encoding).
+__global__ __launch_bounds__(64) void range_init_kernel(int* out) {
+  if (threadIdx.x == 0) {
+    out[0] = 0x7fffffff;
+    out[1] = -0x7fffffff - 1;
+  }
+}
+template <typename VT>
+__global__ __launch_bounds__(256) void range_kernel(const VT* __restrict__ src, long long n, int* out) {
+  int lo = 0x7fffffff, hi = -0x7fffffff - 1;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const int x = (int)src[i];
+    lo = min(lo, x);
+    hi = max(hi, x);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(out, lo);
+    atomicMax(out + 1, hi);
+  }
+}
+
+template <typename VT>
+static int frame_range(rvk_volume* v, const void* dsrc, int& lo, int& hi) {
+  const long long n = (long long)v->dims[0] * v->dims[1] * v->dims[2];
+  int* d = nullptr;
+  RVK_HIP(hipMallocAsync((void**)&d, 2 * sizeof(int), v->ctx->stream));
+  hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(64), 0, v->ctx->stream, d);
+  const int blocks = (int)std::min<long long>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL((range_kernel<VT>), dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc, n, d);
+  int h[2] = {0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, v->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(v->ctx->stream);
+  hipFreeAsync(d, v->ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "frame value range");
+  lo = h[0];
+  hi = h[1];
   return RVK_OK;
 }
 
@@ -425,10 +480,22 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
     if (e != hipSuccess) { hipFree(staging); return hip_fail(e, "hipMemcpyAsync(volume)"); }
     dsrc = staging;
   }
-  int rc = dispatch_cells(v->dtype, v->layout, [&](auto tag) {
-    using CV = decltype(tag);
-    return launch_relayout<typename CV::value_type, CV::kLog2>(v, dsrc, v->frames[frame]);
-  });
+  // float cells always store x-pair differences; 16-bit cells when the
+  // frame's values span less than 32768 (then every difference fits)
+  int rc = RVK_OK;
+  bool diff = v->dtype == RVK_F32;
+  if (v->dtype == RVK_U16 || v->dtype == RVK_I16) {
+    int lo = 0, hi = 0;
+    rc = v->dtype == RVK_U16 ? frame_range<uint16_t>(v, dsrc, lo, hi) : frame_range<int16_t>(v, dsrc, lo, hi);
+    diff = rc == RVK_OK && hi - lo <= 32767;
+  }
+  if (rc == RVK_OK) {
+    v->diff[frame] = diff;
+    rc = dispatch_cells(v->dtype, v->layout, diff, [&](auto tag) {
+      using CV = decltype(tag);
+      return launch_relayout<typename CV::value_type, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);
+    });
+  }
   if (staging) {
     hipError_t e = hipStreamSynchronize(v->ctx->stream);
     hipFree(staging);
@@ -448,7 +515,7 @@ int rvk_volume_compute_gradmag(rvk_volume* v, int frame, float max_gradient) {
   const float inv = 255.0f / max_gradient;
   uint8_t* linear = nullptr;
   RVK_HIP(hipMalloc((void**)&linear, (size_t)v->dims[0] * v->dims[1] * v->dims[2]));
-  rc = dispatch_cells(v->dtype, v->layout, [&](auto tag) { return launch_gradmag<decltype(tag)>(v, frame, inv, linear); });
+  rc = dispatch_frame(v, frame, [&](auto tag) { return launch_gradmag<decltype(tag)>(v, frame, inv, linear); });
   const hipError_t e = hipStreamSynchronize(v->ctx->stream);
   hipFree(linear);
   if (rc == RVK_OK && e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");

@@ -85,12 +85,13 @@ def bench_drr(scene, steps: int, warmup: int):
             "image_mean": float(img.mean())}
 
 
-def bench_compositing(scene, steps: int, warmup: int):
+def bench_compositing(scene, steps: int, warmup: int, gradmag: bool = True):
     """1D (c1) or 2D (c3) compositing caster.  N_samples is the plain march
     (skip rule off) up to opacity termination: one render with the skip rule
     off writes the per-pixel sample count into the steps-taken side output,
     and its sum is the count.  The timed renders use the configured march
-    (skip rule on)."""
+    (skip rule on).  gradmag=False renders the 2D caster with gradients from
+    six gathers instead of the u8 gradient-magnitude volume."""
     import numpy as np
     import torch
 
@@ -103,11 +104,12 @@ def bench_compositing(scene, steps: int, warmup: int):
     two_d = scene.tf2d is not None
     if two_d:
         tf = scene.tf2d
-        vol.compute_gradmag(0, tf["gradmag_max"])
+        if gradmag:
+            vol.compute_gradmag(0, tf["gradmag_max"])
         tfh = abi.TF2D(ctx, tf["tables"], configs.tf2d_params(tf["params"]))
 
         def render(mp, o, aux=None):
-            abi.render_2d(ctx, vol, tfh, view, mp, o, True, aux)
+            abi.render_2d(ctx, vol, tfh, view, mp, o, gradmag, aux)
     else:
         tf = scene.tf1d
         tfh = abi.TF1D(ctx, tf["alpha"], np.stack([tf["r"], tf["g"], tf["b"]], 1), configs.tf1d_params(tf["params"]),

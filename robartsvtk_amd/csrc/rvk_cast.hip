@@ -195,6 +195,8 @@ struct Cast2dArgs {
   const uint32_t* occ;   // opacity bitmaps staged in LDS, or NULL
   const uint32_t* kocc;
   int occ_words;
+  const uint32_t* col;   // column bitmaps (TF, then keyhole TF), staged in LDS
+  int col_words;
   const void* vol;
   const uint8_t* gradmag;
   float gradmag_scale;
@@ -212,15 +214,18 @@ struct Cast2dArgs {
 // table fetch; the cell is read only for samples that contribute.
 template <typename CV, bool SKIP, bool DUAL, bool GMAG, bool PRECISE>
 __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
-  extern __shared__ uint32_t s_occ[];
+  extern __shared__ uint32_t s_lds[];
+  const int cw = a.col_words;
+  for (int i = threadIdx.x; i < (DUAL ? 2 : 1) * cw; i += 256) s_lds[i] = a.col[i];
+  uint32_t* s_occ = s_lds + (DUAL ? 2 : 1) * cw;
   const bool use_occ = a.occ != nullptr;
   if (use_occ) {
     const int w = a.occ_words;
     for (int i = threadIdx.x; i < w; i += 256) s_occ[i] = a.occ[i];
     if (DUAL)
       for (int i = threadIdx.x; i < w; i += 256) s_occ[w + i] = a.kocc[i];
-    __syncthreads();
   }
+  __syncthreads();
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int x, y;
   tile_pixel(tile, a.tiles_x, x, y);
@@ -260,27 +265,33 @@ __global__ __launch_bounds__(256) void cast2d_kernel(Cast2dArgs a) {
     }
     const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
     const float u = mult * (trilinear(vol, g, q.x, q.y, q.z) - lo);
-    V3<float> gr{0.0f, 0.0f, 0.0f};
-    float gidx, gmag;
-    if (GMAG) {
-      gmag = trilinear(Cells<uint8_t, CV::kLog2>{a.gradmag}, g, q.x, q.y, q.z) * a.gradmag_scale;
-      gidx = gmult * (log2f(gmag * gmag + goff) + glo);
-    } else {
-      gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
-      gr.y = (trilinear(vol, g, q.x, q.y + 1.0f, q.z) - trilinear(vol, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
-      gr.z = (trilinear(vol, g, q.x, q.y, q.z + 1.0f) - trilinear(vol, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
-      const float gm2 = gr.x * gr.x + gr.y * gr.y + gr.z * gr.z;
-      gidx = gmult * (log2f(gm2 + goff) + glo);
-      gmag = sqrtf(gm2);
-    }
-    const int ti = tf2d_index(n, u, gidx);
     const Tf2dCell* __restrict__ cells = (DUAL && in_key) ? a.kcells : a.cells;
+    V3<float> gr{0.0f, 0.0f, 0.0f};
+    float gmag = 0.0f;
+    int ti = 0;
     float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (use_occ) {
-      const uint32_t* occ = s_occ + ((DUAL && in_key) ? a.occ_words : 0);
-      if ((occ[ti >> 5] >> (ti & 31)) & 1u) c0 = *(const float4*)&cells[ti].alpha;
-    } else {
-      c0 = *(const float4*)&cells[ti].alpha;
+    // a sample in a TF column without opacity cannot composite, whatever its
+    // gradient: its gradient gathers and table fetch are skipped
+    if (tf2d_column_live(s_lds + ((DUAL && in_key) ? cw : 0), tf2d_axis(n, u))) {
+      float gidx;
+      if (GMAG) {
+        gmag = trilinear(Cells<uint8_t, CV::kLog2>{a.gradmag}, g, q.x, q.y, q.z) * a.gradmag_scale;
+        gidx = gmult * (log2f(gmag * gmag + goff) + glo);
+      } else {
+        gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
+        gr.y = (trilinear(vol, g, q.x, q.y + 1.0f, q.z) - trilinear(vol, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
+        gr.z = (trilinear(vol, g, q.x, q.y, q.z + 1.0f) - trilinear(vol, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
+        const float gm2 = gr.x * gr.x + gr.y * gr.y + gr.z * gr.z;
+        gidx = gmult * (log2f(gm2 + goff) + glo);
+        gmag = sqrtf(gm2);
+      }
+      ti = tf2d_index(n, u, gidx);
+      if (use_occ) {
+        const uint32_t* occ = s_occ + ((DUAL && in_key) ? a.occ_words : 0);
+        if ((occ[ti >> 5] >> (ti & 31)) & 1u) c0 = *(const float4*)&cells[ti].alpha;
+      } else {
+        c0 = *(const float4*)&cells[ti].alpha;
+      }
     }
     const float alpha = c0.x;
     auto accumulate = [&]() {
@@ -498,9 +509,12 @@ int rvk_render_2d(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_tf2d
   a.occ = tf->occ;
   a.kocc = tf->kocc;
   a.occ_words = tf->occ_words;
+  a.col = tf->col;
+  a.col_words = tf->col_words;
   // the keyhole bitmap is needed exactly when the keyhole TF is
   if (tf->kcells && !tf->kocc) a.occ = nullptr;
-  const size_t lds = a.occ ? (size_t)a.occ_words * sizeof(uint32_t) * (tf->kcells ? 2 : 1) : 0;
+  const int ntf = tf->kcells ? 2 : 1;
+  const size_t lds = sizeof(uint32_t) * ((size_t)tf->col_words * ntf + (a.occ ? (size_t)a.occ_words * ntf : 0));
   a.vol = vol->frames[frame];
   a.gradmag = use_gradmag ? vol->gradmag[frame] : nullptr;
   a.gradmag_scale = use_gradmag ? vol->gradmag_scale[frame] : 0.0f;

@@ -31,6 +31,8 @@ struct InExArgs {
   const uint8_t* flags;  // per cell: bit 0 composites, bit 1 excludes
   const uint32_t* occ;   // the same as two bitmaps (occ_words each), or NULL
   int occ_words;
+  const uint32_t* col;   // column bitmap of bit 0 (col_words), staged in LDS
+  int col_words;
   const void* vol;
   uint32_t* out;
   float* depth;
@@ -47,13 +49,15 @@ struct InExArgs {
 // anything is opaque.
 template <typename CV, bool SKIP, bool PRECISE>
 __global__ __launch_bounds__(256) void inex_kernel(InExArgs a) {
-  extern __shared__ uint32_t s_occ[];
+  extern __shared__ uint32_t s_lds[];
+  const int cw = a.col_words;
+  for (int i = threadIdx.x; i < cw; i += 256) s_lds[i] = a.col[i];
+  uint32_t* s_occ = s_lds + cw;
   const bool use_occ = a.occ != nullptr;
   const int w = a.occ_words;
-  if (use_occ) {
+  if (use_occ)
     for (int i = threadIdx.x; i < 2 * w; i += 256) s_occ[i] = a.occ[i];
-    __syncthreads();
-  }
+  __syncthreads();
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int x, y;
   tile_pixel(tile, a.tiles_x, x, y);
@@ -82,20 +86,25 @@ __global__ __launch_bounds__(256) void inex_kernel(InExArgs a) {
   while (k > 0) {
     const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
     const float u = mult * (trilinear(vol, g, q.x, q.y, q.z) - lo);
-    V3<float> gr;
-    gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
-    gr.y = (trilinear(vol, g, q.x, q.y + 1.0f, q.z) - trilinear(vol, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
-    gr.z = (trilinear(vol, g, q.x, q.y, q.z + 1.0f) - trilinear(vol, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
-    const float gm2 = gr.x * gr.x + gr.y * gr.y + gr.z * gr.z;
-    const float gidx = gmult * (log2f(gm2 + goff) + glo);
-    const float gmag = sqrtf(gm2);
-    const int ti = tf2d_index(n, u, gidx);
-    uint32_t fl;
-    if (use_occ) {
-      const uint32_t wi = (uint32_t)ti >> 5, b = (uint32_t)ti & 31u;
-      fl = ((s_occ[wi] >> b) & 1u) | (((s_occ[w + wi] >> b) & 1u) << 1);
-    } else {
-      fl = a.flags[ti];
+    V3<float> gr{0.0f, 0.0f, 0.0f};
+    float gmag = 0.0f;
+    int ti = 0;
+    uint32_t fl = 0u;
+    // no cell of this column composites: the gradient is not needed
+    if (tf2d_column_live(s_lds, tf2d_axis(n, u))) {
+      gr.x = (trilinear(vol, g, q.x + 1.0f, q.y, q.z) - trilinear(vol, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
+      gr.y = (trilinear(vol, g, q.x, q.y + 1.0f, q.z) - trilinear(vol, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
+      gr.z = (trilinear(vol, g, q.x, q.y, q.z + 1.0f) - trilinear(vol, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
+      const float gm2 = gr.x * gr.x + gr.y * gr.y + gr.z * gr.z;
+      const float gidx = gmult * (log2f(gm2 + goff) + glo);
+      gmag = sqrtf(gm2);
+      ti = tf2d_index(n, u, gidx);
+      if (use_occ) {
+        const uint32_t wi = (uint32_t)ti >> 5, b = (uint32_t)ti & 31u;
+        fl = ((s_occ[wi] >> b) & 1u) | (((s_occ[w + wi] >> b) & 1u) << 1);
+      } else {
+        fl = a.flags[ti];
+      }
     }
     // false: the ray ends here, nothing added
     auto accumulate = [&]() -> bool {
@@ -161,6 +170,8 @@ struct DualArgs {
   const uint32_t* occ;  // opacity bitmaps staged in LDS, or NULL
   const uint32_t* kocc;
   int occ_words;
+  const uint32_t* col;  // column bitmaps (TF, then keyhole TF), staged in LDS
+  int col_words;
   const void* vol1;
   const void* vol2;
   uint32_t* out;
@@ -184,15 +195,18 @@ __device__ __forceinline__ int dual_step_count(float num_steps, float j) {
 // jumps over the keyhole window.  Both round the window bounds up/down.
 template <typename CV, typename CV2, bool SKIP, bool KEYTF, bool PRECISE>
 __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
-  extern __shared__ uint32_t s_occ[];
+  extern __shared__ uint32_t s_lds[];
+  const int cw = a.col_words;
+  for (int i = threadIdx.x; i < (KEYTF ? 2 : 1) * cw; i += 256) s_lds[i] = a.col[i];
+  uint32_t* s_occ = s_lds + (KEYTF ? 2 : 1) * cw;
   const bool use_occ = a.occ != nullptr;
   if (use_occ) {
     const int w = a.occ_words;
     for (int i = threadIdx.x; i < w; i += 256) s_occ[i] = a.occ[i];
     if (KEYTF)
       for (int i = threadIdx.x; i < w; i += 256) s_occ[w + i] = a.kocc[i];
-    __syncthreads();
   }
+  __syncthreads();
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int x, y;
   tile_pixel(tile, a.tiles_x, x, y);
@@ -230,21 +244,27 @@ __global__ __launch_bounds__(256) void dual_kernel(DualArgs a) {
     }
     const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, kst, k);
     const float u1 = m1 * (trilinear(vol1, g, q.x, q.y, q.z) - lo1);
-    const float u2 = m2 * (trilinear(vol2, g, q.x, q.y, q.z) - lo2);
-    V3<float> gr;
-    gr.x = (trilinear(vol1, g, q.x + 1.0f, q.y, q.z) - trilinear(vol1, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
-    gr.y = (trilinear(vol1, g, q.x, q.y + 1.0f, q.z) - trilinear(vol1, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
-    gr.z = (trilinear(vol1, g, q.x, q.y, q.z + 1.0f) - trilinear(vol1, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
-    const float gmag = sqrtf(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
-    const int ti = tf2d_index(n, u1, u2);
     const bool kt = KEYTF && in_key;
     const Tf2dCell* __restrict__ cells = kt ? a.kcells : a.cells;
+    V3<float> gr{0.0f, 0.0f, 0.0f};
+    float gmag = 0.0f;
+    int ti = 0;
     float4 c0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (use_occ) {
-      const uint32_t* occ = s_occ + (kt ? a.occ_words : 0);
-      if ((occ[ti >> 5] >> (ti & 31)) & 1u) c0 = *(const float4*)&cells[ti].alpha;
-    } else {
-      c0 = *(const float4*)&cells[ti].alpha;
+    // component 1 picks the column: an empty one needs neither component 2
+    // nor the gradient
+    if (tf2d_column_live(s_lds + (kt ? cw : 0), tf2d_axis(n, u1))) {
+      const float u2 = m2 * (trilinear(vol2, g, q.x, q.y, q.z) - lo2);
+      gr.x = (trilinear(vol1, g, q.x + 1.0f, q.y, q.z) - trilinear(vol1, g, q.x - 1.0f, q.y, q.z)) * g.srecip[0];
+      gr.y = (trilinear(vol1, g, q.x, q.y + 1.0f, q.z) - trilinear(vol1, g, q.x, q.y - 1.0f, q.z)) * g.srecip[1];
+      gr.z = (trilinear(vol1, g, q.x, q.y, q.z + 1.0f) - trilinear(vol1, g, q.x, q.y, q.z - 1.0f)) * g.srecip[2];
+      gmag = sqrtf(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+      ti = tf2d_index(n, u1, u2);
+      if (use_occ) {
+        const uint32_t* occ = s_occ + (kt ? a.occ_words : 0);
+        if ((occ[ti >> 5] >> (ti & 31)) & 1u) c0 = *(const float4*)&cells[ti].alpha;
+      } else {
+        c0 = *(const float4*)&cells[ti].alpha;
+      }
     }
     const float alpha = c0.x;
     auto accumulate = [&]() {
@@ -319,7 +339,9 @@ int rvk_render_2d_inex(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk
   a.flags = tf->inex_flags;
   a.occ = tf->inex_occ;
   a.occ_words = tf->occ_words;
-  const size_t lds = a.occ ? 2 * (size_t)a.occ_words * sizeof(uint32_t) : 0;
+  a.col = tf->inex_col;
+  a.col_words = tf->col_words;
+  const size_t lds = sizeof(uint32_t) * ((size_t)tf->col_words + (a.occ ? 2 * (size_t)a.occ_words : 0));
   a.vol = vol->frames[frame];
   a.out = (uint32_t*)out;
   a.depth = aux ? aux->depth : nullptr;
@@ -364,8 +386,11 @@ int rvk_render_dual(rvk_ctx* ctx, const rvk_volume* vol1, const rvk_volume* vol2
   a.occ = tf->occ;
   a.kocc = tf->kocc;
   a.occ_words = tf->occ_words;
+  a.col = tf->col;
+  a.col_words = tf->col_words;
   if (tf->kcells && !tf->kocc) a.occ = nullptr;
-  const size_t lds = a.occ ? (size_t)a.occ_words * sizeof(uint32_t) * (tf->kcells ? 2 : 1) : 0;
+  const int ntf = tf->kcells ? 2 : 1;
+  const size_t lds = sizeof(uint32_t) * ((size_t)tf->col_words * ntf + (a.occ ? (size_t)a.occ_words * ntf : 0));
   a.vol1 = vol1->frames[frame];
   a.vol2 = vol2->frames[frame];
   a.out = (uint32_t*)out;

@@ -336,13 +336,13 @@ __device__ __forceinline__ float tf1d_lookup(const float* t, int n, float u) {
 }
 
 // tex2D(normalised, point, clamp) cell index g*n + i.
-__device__ __forceinline__ int tf2d_index(int n, float u, float v) {
+__device__ __forceinline__ int tf2d_axis(int n, float u) {
   const float x = fminf(fmaxf(u * (float)n, -1.0f), (float)n);
-  const float y = fminf(fmaxf(v * (float)n, -1.0f), (float)n);
-  const int i = min(max((int)floorf(x), 0), n - 1);
-  const int j = min(max((int)floorf(y), 0), n - 1);
-  return j * n + i;
+  return min(max((int)floorf(x), 0), n - 1);
 }
+__device__ __forceinline__ int tf2d_index(int n, float u, float v) { return tf2d_axis(n, v) * n + tf2d_axis(n, u); }
+// Whether column i of a TF's column bitmap has any opacity.
+__device__ __forceinline__ bool tf2d_column_live(const uint32_t* col, int i) { return (col[i >> 5] >> (i & 31)) & 1u; }
 
 // ---------------------------------------------------------------------------
 // Ray setup (formRays, fused into every caster's prologue).

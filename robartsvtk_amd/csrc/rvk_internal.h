@@ -55,6 +55,12 @@ struct rvk_tf2d {
   uint32_t* occ = nullptr;
   uint32_t* kocc = nullptr;
   int occ_words = 0;
+  // Column bitmaps (bit i = some g has alpha[g*n + i] > 0), col_words =
+  // ceil(n / 32) each: the TF, then the keyhole TF if any.  A sample whose
+  // intensity column is empty cannot composite, so its gradient is not
+  // needed.  Always staged in LDS (n <= 2048: at most 512 B).
+  uint32_t* col = nullptr;
+  int col_words = 0;
   rvk_tf2d_params p{};
   // In/Ex logic (rvk_tf2d_set_inex).  The caster only compares the table
   // with 0 and 0.5, so it keeps per cell bit 0 = (inEx > 0 || alpha > 0)
@@ -62,6 +68,7 @@ struct rvk_tf2d {
   // two bitmaps (occ_words each) staged in LDS.
   uint8_t* inex_flags = nullptr;
   uint32_t* inex_occ = nullptr;
+  uint32_t* inex_col = nullptr;  // column bitmap of flag bit 0 (col_words)
   std::vector<float> alpha_host;  // the opacity table, kept for the flags
 };
 

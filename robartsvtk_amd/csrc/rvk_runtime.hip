@@ -590,6 +590,21 @@ static int upload_occupancy(int n, const float* alpha, uint32_t** dst) {
   return RVK_OK;
 }
 
+// Column bitmaps of `count` n x n tables (index g*n + i): bit i of table t is
+// set when some row has a value > 0 in column i.
+static int upload_columns(int n, const float* const* tabs, int count, uint32_t** dst) {
+  const size_t w = (size_t)(n + 31) / 32;
+  std::vector<uint32_t> bits(w * count, 0u);
+  for (int t = 0; t < count; ++t)
+    for (int g = 0; g < n; ++g)
+      for (int i = 0; i < n; ++i)
+        if (tabs[t][(size_t)g * n + i] > 0.0f) bits[w * t + (i >> 5)] |= 1u << (i & 31);
+  RVK_HIP(hipMalloc((void**)dst, bits.size() * sizeof(uint32_t)));
+  hipError_t e = hipMemcpy(*dst, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { hipFree(*dst); *dst = nullptr; return hip_fail(e, "hipMemcpy(tf2d columns)"); }
+  return RVK_OK;
+}
+
 static int upload_cells(int n, const float* const tables[8], Tf2dCell** dst) {
   for (int i = 0; i < 8; ++i)
     if (!tables[i]) return fail(RVK_ERR_INVALID_ARGUMENT, "2D TF table %d is NULL", i);
@@ -628,11 +643,17 @@ int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const flo
     rc = upload_occupancy(n, tables[0], &t->occ);
     if (rc == RVK_OK && ktables) rc = upload_occupancy(n, ktables[0], &t->kocc);
   }
+  if (rc == RVK_OK) {
+    const float* alphas[2] = {tables[0], ktables ? ktables[0] : nullptr};
+    t->col_words = (n + 31) / 32;
+    rc = upload_columns(n, alphas, ktables ? 2 : 1, &t->col);
+  }
   if (rc != RVK_OK) {
     if (t->cells) hipFree(t->cells);
     if (t->kcells) hipFree(t->kcells);
     if (t->occ) hipFree(t->occ);
     if (t->kocc) hipFree(t->kocc);
+    if (t->col) hipFree(t->col);
     delete t;
     return rc;
   }
@@ -650,6 +671,7 @@ int rvk_tf2d_set_inex(rvk_tf2d* t, const float* inex) {
   RVK_HIP(hipStreamSynchronize(t->ctx->stream));
   if (t->inex_flags) { hipFree(t->inex_flags); t->inex_flags = nullptr; }
   if (t->inex_occ) { hipFree(t->inex_occ); t->inex_occ = nullptr; }
+  if (t->inex_col) { hipFree(t->inex_col); t->inex_col = nullptr; }
   if (!inex) return RVK_OK;
   const size_t nn = (size_t)t->n * t->n;
   std::vector<uint8_t> flags(nn);
@@ -661,6 +683,17 @@ int rvk_tf2d_set_inex(rvk_tf2d* t, const float* inex) {
     hipFree(t->inex_flags);
     t->inex_flags = nullptr;
     return hip_fail(e, "hipMemcpy(inex flags)");
+  }
+  {
+    std::vector<float> live(nn);
+    for (size_t c = 0; c < nn; ++c) live[c] = (flags[c] & 1u) ? 1.0f : 0.0f;
+    const float* tabs[1] = {live.data()};
+    const int rc = upload_columns(t->n, tabs, 1, &t->inex_col);
+    if (rc != RVK_OK) {
+      hipFree(t->inex_flags);
+      t->inex_flags = nullptr;
+      return rc;
+    }
   }
   if (t->occ_words) {
     const size_t w = (size_t)t->occ_words;
@@ -690,6 +723,8 @@ int rvk_tf2d_destroy(rvk_tf2d* t) {
   if (t->kocc) hipFree(t->kocc);
   if (t->inex_flags) hipFree(t->inex_flags);
   if (t->inex_occ) hipFree(t->inex_occ);
+  if (t->inex_col) hipFree(t->inex_col);
+  if (t->col) hipFree(t->col);
   delete t;
   return RVK_OK;
 }

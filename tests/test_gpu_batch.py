@@ -83,6 +83,19 @@ def test_autotune_and_graph_replays(gpu_ctx):
         plan.render(out)
         torch.cuda.synchronize()
         assert torch.equal(out, eager), rep
+    # re-uploading the frame with values spanning more than 32767 changes its
+    # cell encoding: the captured graph is not replayed for it, the render is
+    # launched directly and matches a fresh plan
+    wide = np.clip(s.volume.astype(np.int32) * 20, -32768, 32767).astype(np.int16)
+    assert int(wide.max()) - int(wide.min()) > 32767
+    vol.set_frame(0, wide)
+    plan.render(out)
+    fresh = abi.DrrPlan(gpu_ctx, vol, s.views, mp, dp)
+    ref = torch.zeros_like(out)
+    fresh.render(ref)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref) and not torch.equal(out, eager)
+    fresh.close()
     plan.close()
 
 

@@ -78,6 +78,26 @@ def test_drr_all_voxel_types(gpu_ctx, dtype):
     _drr_check(gpu_ctx, vol, s.spacing, s.views[0], s.march, drr)
 
 
+@pytest.mark.parametrize("dtype", ["u16", "i16"])
+def test_drr_wide_range_16bit_frames(gpu_ctx, dtype):
+    """16-bit frames whose values span 32768 or more keep plain (a, b) cells
+    (their x-pair differences do not fit 16 bits); a value range just below
+    keeps difference cells.  Both render as the oracle does."""
+    s = configs.scene_c2(scale=0.125)
+    hu = s.volume.astype(np.float64)
+    drr = dict(s.drr)
+    for stretch in (16.0, 7.9):  # HU span ~2500 x stretch: above and below 32767
+        if dtype == "u16":
+            vol = np.clip(np.rint((hu + 1000.0) * stretch), 0, 65535).astype(np.uint16)
+            drr.update(slope=2e-5 / stretch, intercept=0.0)
+        else:
+            vol = np.clip(np.rint(hu * stretch), -32768, 32767).astype(np.int16)
+            drr.update(slope=2e-5 / stretch, intercept=0.02)
+        span = int(vol.max()) - int(vol.min())
+        assert (span > 32767) == (stretch > 10)
+        _drr_check(gpu_ctx, vol, s.spacing, s.views[0], s.march, drr)
+
+
 @pytest.mark.parametrize("skip", [0, 1])
 @pytest.mark.parametrize("precise", [0, 1])
 @pytest.mark.parametrize("jitter", [0, 1])

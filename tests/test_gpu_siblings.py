@@ -168,6 +168,26 @@ def test_dual_parity(gpu_ctx, skip, keytf, precise):
     h2.close()
 
 
+def test_dual_mixed_cell_encodings(gpu_ctx):
+    """Component 1 spans more than 32767 (plain cells), component 2 less
+    (difference cells): the dual caster reads each in its own encoding."""
+    s, view, v2, tf = _dual_scene()
+    v1 = (s.volume.astype(np.uint32) * 16).clip(0, 65535).astype(np.uint16)
+    assert int(v1.max()) - int(v1.min()) > 32767 and int(v2.max()) - int(v2.min()) <= 32767
+    tf = dict(tf, params=dict(tf["params"], intensity_multiplier=float(np.float32(1.0 / 65520.0))))
+    march = dict(s.march, term_T=1.0 / 32)
+    h1 = H.gpu_volume(gpu_ctx, v1, s.spacing)
+    h2 = H.gpu_volume(gpu_ctx, v2, s.spacing)
+    g = H.gpu_dual(gpu_ctx, h1, h2, tf, view, march)
+    vd = configs.view_dict(view)
+    o32, _, _ = orc.cast_dual(v1, v2, s.spacing, tf, vd, march, "f32")
+    o64, _, _ = orc.cast_dual(v1, v2, s.spacing, tf, vd, dict(march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, "dual mixed encodings")
+    assert (o32[..., 3] > 0).mean() > 0.05
+    h1.close()
+    h2.close()
+
+
 def test_siblings_identical_across_cell_layouts(gpu_ctx):
     s, view, v2, tf = _dual_scene()
     _keyhole(view, s.dims[0])

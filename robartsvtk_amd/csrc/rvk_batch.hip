@@ -402,6 +402,8 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
   rr.p0z = (r.start.z + j * r.inc.z) + 0.5f;
   rr.ix = r.inc.x; rr.iy = r.inc.y; rr.iz = r.inc.z;
   rr.n = step_count(r.num_steps - j);
+  // the rays the precise forward march takes no samples of (drr_march)
+  if (!samples_in_cells(g, V3<float>{rr.p0x, rr.p0y, rr.p0z}, r.inc, rr.n)) rr.n = 0;
   rr.ci = a.b.m.intercept * -ds;
   rr.cs = a.b.m.slope * -ds;
   // d loss / d mu at a sample = (d loss / d L) * ds * slope, with dL/dI = -I for exp(-L)

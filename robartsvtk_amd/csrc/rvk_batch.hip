@@ -48,6 +48,7 @@ struct BatchArgs {
 // Accumulates A = sum ds*slope*grad mu and B = sum (j+t) ds*slope*grad mu over
 // the accumulated samples of one ray (see the pose adjoint below).
 struct PoseVisit {
+  static constexpr bool kFlatRuns = true;  // a zero gradient adds nothing
   float ncs, j;
   V3<float>& A;
   V3<float>& B;
@@ -87,14 +88,14 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
   const Ray r = form_ray(a.g, bv.v, a.step_len, x, y);
   const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
   if (!JAC) {
-    a.out[pi] = drr_march<CV, SKIP, PRECISE>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
+    a.out[pi] = drr_march<CV, SKIP, PRECISE, false, true>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
     return;
   }
   const float ds = sqrtf(r.inc.x * r.inc.x * a.g.sp[0] * a.g.sp[0] + r.inc.y * r.inc.y * a.g.sp[1] * a.g.sp[1] +
                          r.inc.z * r.inc.z * a.g.sp[2] * a.g.sp[2]);
   V3<float> A{0, 0, 0}, B{0, 0, 0};
   PoseVisit visit{ds * a.m.slope, ray_offset(a.m.jitter, x, y), A, B};
-  a.out[pi] = drr_march<CV, SKIP, true, true>(cells_of<CV>(a.vol), a.g, a.m, r, x, y, visit);
+  a.out[pi] = drr_march<CV, SKIP, true, true, true>(cells_of<CV>(a.vol), a.g, a.m, r, x, y, visit);
   const bool any = A.x != 0.0f || A.y != 0.0f || A.z != 0.0f || B.x != 0.0f || B.y != 0.0f || B.z != 0.0f;
 #pragma unroll 1
   for (int i = 0; i < 6; ++i) {
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(256) void adjoint_pose_kernel(AdjArgs a) {
     V3<float> A{0, 0, 0}, B{0, 0, 0};
     const float jt = ray_offset(a.b.m.jitter, x, y);
     PoseVisit visit{ds * a.b.m.slope, jt, A, B};
-    const float I = drr_march<CV, SKIP, true, true>(cells_of<CV>(a.b.vol), g, a.b.m, r, x, y, visit);
+    const float I = drr_march<CV, SKIP, true, true, true>(cells_of<CV>(a.b.vol), g, a.b.m, r, x, y, visit);
     const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
     const float dLdI = dloss_at(a.dldi, a.stats, a.img, a.tgt, v, pi);
     // dI/dL = -I for exp(-L) output, 1 for L output
@@ -595,6 +596,9 @@ BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
   a.m.slope = p->drr.ct_slope;
   a.m.offset = p->drr.ct_offset;
   a.m.output = p->drr.output;
+  a.m.ranges = p->vol->ranges[p->frame];
+  a.m.rbx = p->vol->nrb[0];
+  a.m.rby = p->vol->nrb[1];
   a.m.ref_term = p->drr.ref_termination;
   a.m.jitter = p->march.jitter;
   a.step_len = min_spacing(p->vol) * p->march.step_scale;

@@ -110,6 +110,11 @@ template <int LOG2> struct CellOrder {
 constexpr int kCellLayoutPlain = 0;
 constexpr int kCellLayoutBrick4 = 2;
 
+// Range bricks: kRangeEdge^3 cells (cells [8b, 8b + 8) per axis, i.e. the
+// voxels [8b - 1, 8b + 7] their corners reach).
+constexpr int kRangeLog2 = 3;
+constexpr int kRangeEdge = 1 << kRangeLog2;
+
 // A cell volume of voxel type T in cell order LOG2 (what the kernels are
 // templated on; `d` is the device array of cells).  DIFF cells store each
 // x-pair as (a, b - a) instead of (a, b): the first lerp level then needs no

@@ -30,6 +30,10 @@ struct rvk_volume {
   uint8_t diff[RVK_MAX_FRAMES] = {};    // 1: cells store x-pair differences (see Cells)
   uint8_t* gradmag[RVK_MAX_FRAMES] = {};  // u8 cell volumes
   float gradmag_scale[RVK_MAX_FRAMES] = {};
+  // Value range (min, max) of every brick of kRangeEdge^3 cells, for the DRR
+  // march's empty-space skipping; nrb = ceil((dims + 2) / kRangeEdge).
+  float2* ranges[RVK_MAX_FRAMES] = {};
+  int nrb[3] = {0, 0, 0};
 };
 
 struct rvk_tf1d {

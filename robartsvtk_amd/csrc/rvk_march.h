@@ -106,9 +106,96 @@ __device__ __forceinline__ bool samples_in_cells(const VolGeom& g, const V3<floa
 struct DrrMarchArgs {
   float intercept, slope, offset;
   int output, ref_term, jitter;
+  const float2* ranges;  // per range brick (min, max) of the frame, or NULL
+  int rbx, rby;          // range-brick grid extent in x and y
 };
 
+// Run skipping of the precise march.  Each range brick (kRangeEdge^3 cells)
+// has the (min, max) of every value its cells can interpolate.  After a
+// sample at cell coordinate q:
+//  - if no value of q's brick can contribute (alpha = fma(cs, v, ci) >= 0 at
+//    the bound that minimises it: alpha is monotone in v), the samples the
+//    march takes next inside the brick do not contribute either;
+//  - if the brick is constant (min == max: every corner equal, so every
+//    sample in it interpolates that value exactly) and q contributed, the
+//    next samples contribute the same alpha.
+// Either way the march's state after such a run is known in closed form, so
+// the run's gathers are skipped (a contributing run still adds its alphas
+// one by one, in order: the sum is the same to the bit).
+struct RangeWalk {
+  float hx, hy, hz;  // the brick face a ray leaves through: kRangeEdge (ahead) or 0
+  float rx, ry, rz;  // 1 / inc, +inf where inc = 0
+};
+__device__ __forceinline__ RangeWalk range_walk(const V3<float>& inc) {
+  const float e = (float)kRangeEdge, inf = __builtin_huge_valf();
+  return RangeWalk{inc.x >= 0.0f ? e : 0.0f, inc.y >= 0.0f ? e : 0.0f, inc.z >= 0.0f ? e : 0.0f,
+                   inc.x != 0.0f ? 1.0f / inc.x : inf, inc.y != 0.0f ? 1.0f / inc.y : inf,
+                   inc.z != 0.0f ? 1.0f / inc.z : inf};
+}
+
+// How many of the samples t1, t1 + stride, ... before lim lie in range brick
+// (bx, by, bz), which holds the sample before t1.  The estimate from the
+// brick's exit time is confirmed on its last sample, at the exact position
+// the march computes: positions are monotone in t per axis, so when the
+// sample before t1 and the last one lie in the (box-shaped) brick, all
+// between do.  t1 and lim are integers, so the cap at lim is exact.
+__device__ __forceinline__ int brick_run(int bx, int by, int bz, const V3<float>& p0c, const V3<float>& inc,
+                                         const RangeWalk& w, float t1, float lim, float stride) {
+  const float e = (float)kRangeEdge;
+  const float tx = (fmaf((float)bx, e, w.hx) - p0c.x) * w.rx;
+  const float ty = (fmaf((float)by, e, w.hy) - p0c.y) * w.ry;
+  const float tz = (fmaf((float)bz, e, w.hz) - p0c.z) * w.rz;
+  const float te = fminf(fminf(tx, ty), fminf(tz, lim));
+  const float inv = stride == 1.0f ? 1.0f : 0.5f;
+  int m = (int)ceilf((te - t1) * inv);
+  for (int k = 0; k < 2 && m > 0; ++k, --m) {
+    const float tl = fmaf(stride, (float)(m - 1), t1);
+    const int cx = (int)fmaf(tl, inc.x, p0c.x), cy = (int)fmaf(tl, inc.y, p0c.y), cz = (int)fmaf(tl, inc.z, p0c.z);
+    if ((cx >> kRangeLog2) == bx && (cy >> kRangeLog2) == by && (cz >> kRangeLog2) == bz) return m;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ float2 range_brick(const DrrMarchArgs& a, const V3<float>& q, int& bx, int& by, int& bz) {
+  bx = (int)q.x >> kRangeLog2;
+  by = (int)q.y >> kRangeLog2;
+  bz = (int)q.z >> kRangeLog2;
+  return a.ranges[(bz * a.rby + by) * a.rbx + bx];
+}
+
+// Non-contributing samples to pass over after the one at q (the next at t1).
+__device__ __forceinline__ int empty_run(const DrrMarchArgs& a, const V3<float>& q, const V3<float>& p0c,
+                                         const V3<float>& inc, const RangeWalk& w, float cs, float ci, float t1,
+                                         float lim, float stride) {
+  int bx, by, bz;
+  const float2 r = range_brick(a, q, bx, by, bz);
+  if (!(fmaf(cs, cs <= 0.0f ? r.y : r.x, ci) >= 0.0f)) return 0;
+  return brick_run(bx, by, bz, p0c, inc, w, t1, lim, stride);
+}
+
+// Contributing run after the sample at q (alpha; next sample at t): adds the
+// run's alphas to acc and advances t.  Returns true when acc reached stop_at
+// inside the run (the march ends there, as it would have).
+template <typename Acc>
+__device__ __forceinline__ bool flat_run(const DrrMarchArgs& a, const V3<float>& q, const V3<float>& p0c,
+                                         const V3<float>& inc, const RangeWalk& w, float cs, float ci, float alpha,
+                                         float& t, float lim, Acc& acc, Acc stop_at) {
+  int bx, by, bz;
+  const float2 r = range_brick(a, q, bx, by, bz);
+  if (!(r.x == r.y && fmaf(cs, r.x, ci) == alpha)) return false;
+  const int m = brick_run(bx, by, bz, p0c, inc, w, t, lim, 1.0f);
+  for (int i = 0; i < m; ++i) {
+    acc += (Acc)alpha;
+    if (acc <= stop_at) return true;
+  }
+  t += (float)m;
+  return false;
+}
+
+// Visitors declare whether a sample with a zero gradient may go unvisited
+// (kFlatRuns): samples of constant bricks have gradient +0.
 struct NoVisit {
+  static constexpr bool kFlatRuns = true;
   __device__ __forceinline__ void operator()(const V3<float>&, float, float, const V3<float>&) const {}
 };
 
@@ -117,10 +204,15 @@ struct NoVisit {
 // grad)` is called for every accumulated sample, with c its sample cell
 // coordinate (c = p0 + 0.5 + t*inc in PRECISE mode); with VG the trilinear
 // gradient at c comes from the same gathers as the value (the adjoint marches
-// through this very control flow).
-template <typename CV, bool SKIP, bool PRECISE, bool VG = false, typename F = NoVisit>
+// through this very control flow).  RUNS (precise mode only) takes the runs
+// above when the volume frame has brick ranges (a.ranges); the result is the
+// same to the bit.  Runs pay where the rays of a wave skip together (many
+// views in one launch); a single view's launch is ordered for L2 locality,
+// which runs unbalance, and marches without them.
+template <typename CV, bool SKIP, bool PRECISE, bool VG = false, bool RUNS = false, typename F = NoVisit>
 __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMarchArgs& a,
                                            const Ray& r, int x, int y, F&& visit = F()) {
+  using FV = typename std::remove_reference<F>::type;
   const bool ref_mode = a.output == RVK_DRR_RGBA8_REF;
   const float lg = logf(256.0f * a.offset);
   // PRECISE sums the line integral in double: a converted f32 sample and one
@@ -147,12 +239,20 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
   const float tend = (float)kst, kw0 = (float)(kst - es), kw1 = (float)(kst - ee);
   float t = 0.0f;
   int sa = 0, sb = 0;
+  const bool runs = RUNS && PRECISE && a.ranges != nullptr;
+  const RangeWalk w = range_walk(inc);
+  // a contributing run must end before the keyhole window
+  const float flat_lim = fminf(kw0, tend);
+  // a repeated alpha is the cheap hint of a constant brick
+  float alpha_prev = __builtin_nanf("");
   while (t < tend) {
     const bool in_key = (t >= kw0 && t < kw1);
     const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, t);
     V3<float> gr{0.0f, 0.0f, 0.0f};
     const float alpha = fmaf(cs, VG ? trilinear_vg<!PRECISE>(vol, g, q.x, q.y, q.z, gr)
                                     : trilinear<!PRECISE>(vol, g, q.x, q.y, q.z), ci);
+    const bool repeat = alpha == alpha_prev;
+    alpha_prev = alpha;
     const float ts = t;
     if (SKIP) {
       // t stays an exact integer, so one add of +-1 or 2 equals the
@@ -169,6 +269,10 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
         if (VG && !sa) visit(q, ts, alpha, gr);
         acc += (Acc)(sa ? 0.0f : alpha);
         if (acc <= stop_at) break;
+        // each further sample of a contributing run steps 1 (sa = sb = 0)
+        if (FV::kFlatRuns && runs && repeat && !sa) {
+          if (flat_run(a, q, p0c, inc, w, cs, ci, alpha, t, t >= kw1 ? tend : flat_lim, acc, stop_at)) break;
+        }
       } else {
         if (!PRECISE) {
           if (!sa) step(p, inc);
@@ -177,6 +281,14 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
         t += sa ? 1.0f : 2.0f;
         sb = !sa;
         sa = 0;
+        // each further non-contributing sample steps 2 and leaves sb = 1
+        if (runs) {
+          const int m = empty_run(a, q, p0c, inc, w, cs, ci, t, tend, 2.0f);
+          if (m > 0) {
+            t += 2.0f * (float)m;
+            sb = 1;
+          }
+        }
       }
     } else {
       if (!PRECISE) step(p, inc);
@@ -185,6 +297,11 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
         acc += (Acc)alpha;
         if (VG) visit(q, ts, alpha, gr);
         if (acc <= stop_at) break;
+        if (FV::kFlatRuns && runs && repeat) {
+          if (flat_run(a, q, p0c, inc, w, cs, ci, alpha, t, t >= kw1 ? tend : flat_lim, acc, stop_at)) break;
+        }
+      } else if (runs) {
+        t += (float)empty_run(a, q, p0c, inc, w, cs, ci, t, tend, 1.0f);
       }
     }
   }

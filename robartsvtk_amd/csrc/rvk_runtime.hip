@@ -204,6 +204,36 @@ static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
   return RVK_OK;
 }
 
+// (min, max) of the voxels each range brick's cells reach.  The y and z
+// lerps round the difference of their operands, so an interpolated value may
+// leave the voxels' range by a few ulp of the largest magnitude: the bounds
+// of a non-constant brick are widened by 2^-20 of it.  A constant brick
+// interpolates its value exactly and keeps min == max.  A brick with a
+// non-finite voxel gets (-inf, +inf): no run is ever taken through it.
+template <typename VT>
+__global__ __launch_bounds__(256) void brick_range_kernel(const VT* __restrict__ src, int nx, int ny, int nz, int rbx,
+                                                          int rby, int rbz, float2* __restrict__ out) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= rbx * rby * rbz) return;
+  const int bx = b % rbx, by = (b / rbx) % rby, bz = b / (rbx * rby);
+  const int x0 = max(bx * kRangeEdge - 1, 0), x1 = min(bx * kRangeEdge + kRangeEdge - 1, nx - 1);
+  const int y0 = max(by * kRangeEdge - 1, 0), y1 = min(by * kRangeEdge + kRangeEdge - 1, ny - 1);
+  const int z0 = max(bz * kRangeEdge - 1, 0), z1 = min(bz * kRangeEdge + kRangeEdge - 1, nz - 1);
+  const float inf = __builtin_huge_valf();
+  float lo = inf, hi = -inf;
+  bool finite = true;
+  for (int k = z0; k <= z1; ++k)
+    for (int j = y0; j <= y1; ++j)
+      for (int i = x0; i <= x1; ++i) {
+        const float v = (float)src[(long long)i + (long long)j * nx + (long long)k * nx * ny];
+        finite = finite && isfinite(v);
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+      }
+  const float w = hi > lo ? fmaxf(fabsf(lo), fabsf(hi)) * 0x1p-20f : 0.0f;
+  out[b] = finite ? make_float2(lo - w, hi + w) : make_float2(-inf, inf);
+}
+
 // Value range of a 16-bit frame (decides its cell encoding).
 __global__ __launch_bounds__(64) void range_init_kernel(int* out) {
   if (threadIdx.x == 0) {
@@ -264,6 +294,7 @@ static int launch_gradmag(rvk_volume* v, int frame, float inv_scale, uint8_t* li
 static int size_cells(rvk_volume* v) {
   const int edge = 1 << v->layout;
   for (int a = 0; a < 3; ++a) v->ncb[a] = (v->dims[a] + 2 + edge - 1) / edge;
+  for (int a = 0; a < 3; ++a) v->nrb[a] = (v->dims[a] + 2 + kRangeEdge - 1) / kRangeEdge;
   const size_t bc = (size_t)1 << (3 * v->layout);
   v->n_cells = bc * v->ncb[0] * v->ncb[1] * v->ncb[2];
   const size_t slab = bc * v->ncb[0] * v->ncb[1];
@@ -493,7 +524,17 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
     v->diff[frame] = diff;
     rc = dispatch_cells(v->dtype, v->layout, diff, [&](auto tag) {
       using CV = decltype(tag);
-      return launch_relayout<typename CV::value_type, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);
+      using VT = typename CV::value_type;
+      int r = launch_relayout<VT, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);
+      if (r == RVK_OK) {
+        const int nb = v->nrb[0] * v->nrb[1] * v->nrb[2];
+        if (!v->ranges[frame]) RVK_HIP(hipMalloc((void**)&v->ranges[frame], sizeof(float2) * (size_t)nb));
+        hipLaunchKernelGGL((brick_range_kernel<VT>), dim3((nb + 255) / 256), dim3(256), 0, v->ctx->stream,
+                           (const VT*)dsrc, v->dims[0], v->dims[1], v->dims[2], v->nrb[0], v->nrb[1], v->nrb[2],
+                           v->ranges[frame]);
+        RVK_HIP(hipGetLastError());
+      }
+      return r;
     });
   }
   if (staging) {
@@ -529,6 +570,7 @@ int rvk_volume_destroy(rvk_volume* v) {
   for (int f = 0; f < RVK_MAX_FRAMES; ++f) {
     if (v->frames[f]) hipFree(v->frames[f]);
     if (v->gradmag[f]) hipFree(v->gradmag[f]);
+    if (v->ranges[f]) hipFree(v->ranges[f]);
   }
   delete v;
   return RVK_OK;

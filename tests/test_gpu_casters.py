@@ -109,6 +109,74 @@ def test_drr_march_modes(gpu_ctx, skip, precise, jitter):
     _drr_check(gpu_ctx, s.volume, s.spacing, s.views[0], march, s.drr, tol_f64=tol64)
 
 
+def _sparse_volume(dtype, sign, dims=(61, 45, 53), seed=11):
+    """A background whose samples contribute nothing (alpha >= 0, exactly 0
+    for f32) holding isolated contributing voxels and one small dense block,
+    with its slope/intercept."""
+    rng = np.random.default_rng(seed)
+    shape = dims[::-1]
+    hot = rng.random(shape) < 0.002
+    hot[20:24, 10:13, 30:36] = True
+    val = rng.uniform(0.3, 1.0, shape)
+    if dtype == "f32":
+        vol = np.where(hot, sign * val, 0.0).astype(np.float32)
+        drr = dict(slope=float(sign), intercept=0.0)
+    elif dtype == "f32c":  # the background contributes one constant alpha; rays may terminate in it
+        vol = np.where(hot, sign * val, sign * 0.1).astype(np.float32)
+        drr = dict(slope=float(sign), intercept=0.0, ref_termination=1)
+    elif dtype == "i16":  # mu = 2e-5 (v + 1024) >= 0 from v = -1024 up
+        vol = np.where(hot, -1024 + sign * np.rint(3000 * val), -1024).astype(np.int16)
+        drr = dict(slope=sign * 2e-5, intercept=sign * 2e-5 * 1024)
+    else:  # u8, mu = sign (v - 128) / 500
+        vol = np.where(hot, 128 + sign * np.rint(120 * val), 128).astype(np.uint8)
+        drr = dict(slope=sign / 500.0, intercept=-sign * 128 / 500.0)
+    return vol, drr
+
+
+@pytest.mark.parametrize("dtype,sign", [("f32", 1), ("f32", -1), ("f32c", 1), ("i16", 1), ("i16", -1), ("u8", -1)])
+@pytest.mark.parametrize("layout", [abi.CELLS_PLAIN, abi.CELLS_BRICK4])
+def test_drr_sparse_volumes(gpu_ctx, dtype, sign, layout):
+    """Mostly empty volumes: the batched precise march passes over runs of
+    samples that cannot contribute (or that contribute one constant alpha)
+    and must give what the single-view march, which takes every sample,
+    gives: with either skip rule, slope sign (the upper or the lower bound of
+    a region's values decides), ragged dims, and perspective and
+    axis-parallel views (rays with zero direction components)."""
+    dims, spacing = (61, 45, 53), (1.0, 1.2, 0.9)
+    vol, d = _sparse_volume(dtype, sign, dims)
+    drr = {"offset": 1.0, "output": abi.DRR_F32_L, "ref_termination": 0, "tint": (0, 0, 0, 0)}
+    drr.update(d)
+    c = configs.volume_center(dims, spacing)
+    views = []
+    for k, ang in enumerate((0.0, 0.7, 2.3)):
+        cam = configs.drr_camera(c, ang, 0, sod=300.0, half_extent=60.0)
+        views.append(abi.compute_view(cam, 67 + 8 * k, 49, spacing=spacing))
+    along_x = abi.camera(c + np.array([300.0, 0, 0]), c, (0, 0, 1), 30.0, (200.0, 400.0), parallel=True,
+                         parallel_scale=30.0)
+    views.append(abi.compute_view(along_x, 65, 47, spacing=spacing))
+    import torch
+    vh = abi.Volume.from_array(gpu_ctx, vol, spacing, layout=layout)
+    try:
+        for skip in (0, 1):
+            march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": skip, "precise": 1}
+            plan = abi.DrrPlan(gpu_ctx, vh, views, configs.march_params(march), configs.drr_params(drr))
+            packed = torch.full((plan.total,), float("nan"), dtype=torch.float32, device="cuda")
+            plan.render(packed)
+            torch.cuda.synchronize()
+            batched = [b.cpu().numpy() for b in plan.split(packed)]
+            plan.close()
+            for i, view in enumerate(views):
+                g = H.gpu_drr(gpu_ctx, vh, view, march, drr)
+                o32 = orc.drr(vol, spacing, configs.view_dict(view), march, drr, "f32")
+                assert np.abs(o32).max() > 0.1  # the rays meet contributing voxels
+                err = np.abs(g - o32).max()
+                print(f"view {i} skip {skip}: max |L - L_f32| {err:.3e}")
+                assert err <= 1e-5 * max(1.0, np.abs(o32).max()), (i, skip, err)
+                assert np.array_equal(batched[i], g), (i, skip, np.abs(batched[i] - g).max())
+    finally:
+        vh.close()
+
+
 @pytest.mark.parametrize("output", [abi.DRR_RGBA8_REF, abi.DRR_F32_EXP_NEG_L, abi.DRR_F32_L])
 @pytest.mark.parametrize("ref_term", [0, 1])
 def test_drr_outputs_and_termination(gpu_ctx, output, ref_term):

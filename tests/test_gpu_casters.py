@@ -154,6 +154,14 @@ def test_drr_sparse_volumes(gpu_ctx, dtype, sign, layout):
     along_x = abi.camera(c + np.array([300.0, 0, 0]), c, (0, 0, 1), 30.0, (200.0, 400.0), parallel=True,
                          parallel_scale=30.0)
     views.append(abi.compute_view(along_x, 65, 47, spacing=spacing))
+    # a keyhole: contributing runs must stop at its window
+    kv = abi.compute_view(configs.drr_camera(c, 0.7, 0, sod=300.0, half_extent=60.0), 71, 45, spacing=spacing)
+    n = dims[0]
+    key = [(1, 0, 0.01, -0.35 * n), (-1, 0, 0.01, 0.65 * n), (0, 1, 0.01, -0.3 * n), (0, -1, 0.01, 0.7 * n),
+           (0.01, 0, 1, -0.3 * n), (0.01, 0, -1, 0.6 * n)]
+    kv.n_keyhole_planes = 6
+    kv.keyhole_planes[:] = [float(x) for p in key for x in p]
+    views.append(kv)
     import torch
     vh = abi.Volume.from_array(gpu_ctx, vol, spacing, layout=layout)
     try:

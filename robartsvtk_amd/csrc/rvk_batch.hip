@@ -541,7 +541,7 @@ struct rvk_drr_plan {
   const float* g_tgt = nullptr;
   float* g_loss = nullptr;
   float* g_grad = nullptr;
-  int g_diff = 0;  // cell encoding of the frame when the graph was captured
+  int g_enc = 0;  // cell encoding of the frame when the graph was captured
 };
 
 namespace {
@@ -865,7 +865,7 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
 int rvk_drr_plan_render(rvk_drr_plan* p, float* out) {
   if (!p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && p->g_out == out && !p->g_tgt && p->g_diff == p->vol->diff[p->frame]) {
+  if (p->exec && p->g_out == out && !p->g_tgt && p->g_enc == p->vol->enc[p->frame]) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }
@@ -895,7 +895,7 @@ int rvk_drr_plan_capture(rvk_drr_plan* p, float* out, const float* tgt, float* l
   p->g_tgt = tgt;
   p->g_loss = loss;
   p->g_grad = grad;
-  p->g_diff = p->vol->diff[p->frame];
+  p->g_enc = p->vol->enc[p->frame];
   return RVK_OK;
 }
 
@@ -904,7 +904,7 @@ int rvk_drr_plan_register(rvk_drr_plan* p, float* out, const float* tgt, float* 
   if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
   RVK_HIP(hipSetDevice(p->ctx->device));
   if (p->exec && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad &&
-      p->g_diff == p->vol->diff[p->frame]) {
+      p->g_enc == p->vol->enc[p->frame]) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }

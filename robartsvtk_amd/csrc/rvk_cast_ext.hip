@@ -400,10 +400,15 @@ int rvk_render_dual(rvk_ctx* ctx, const rvk_volume* vol1, const rvk_volume* vol2
   const int nblk = launch_grid(view, a.tiles_x);
   const bool skip = march->skip_rule != 0, key = tf->kcells != nullptr, pr = march->precise != 0;
   // the two components share voxel type and cell order, but each frame has
-  // its own cell encoding
-  return dispatch_frame(vol1, frame, [&](auto tag1) {
+  // its own cell encoding; binary16-base frames are re-encoded as plain
+  // differences first (same values), which keeps the kernel pairs to those
+  // of one voxel type
+  rc = frame_drop_half(vol1, frame, ctx->stream);
+  if (!rc) rc = frame_drop_half(vol2, frame, ctx->stream);
+  if (rc) return rc;
+  return dispatch_frame<false>(vol1, frame, [&](auto tag1) {
     using CV = decltype(tag1);
-    return dispatch_frame(vol2, frame, [&](auto tag2) {
+    return dispatch_frame<false>(vol2, frame, [&](auto tag2) {
       using CV2 = decltype(tag2);
       if constexpr (std::is_same<typename CV::value_type, typename CV2::value_type>::value &&
                     CV::kLog2 == CV2::kLog2) {

@@ -245,6 +245,42 @@ template <typename W, typename V4> struct CellLoad32 {
   }
 };
 template <> struct CellLoad<float> : CellLoad32<float, float4> {};
+
+// 16-bit frames whose values all lie in [-2048, 2048] store each x-pair's
+// base as an IEEE binary16 (exact for those integers) and its difference as
+// int16 (always difference-encoded).  The first lerp level then converts the
+// base inside its fma (v_fma_mix_f32) instead of with a separate convert;
+// the values are the same as those of int16 or uint16 cells.
+struct Half16 {
+  uint16_t bits;
+  __device__ __forceinline__ operator float() const { return (float)__builtin_bit_cast(_Float16, bits); }
+};
+template <> struct CellLoad<Half16> {
+  using Raw = uint4;
+  static __device__ __forceinline__ Raw raw(const Half16* __restrict__ d, uint32_t c) {
+    return *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ float base(uint32_t q) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(q & 0xffffu));
+  }
+  static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
+    const uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[2 * i] = base(q[i]);
+      v[2 * i + 1] = v[2 * i] + (float)((int)q[i] >> 16);
+    }
+  }
+  template <bool DIFF> static __device__ __forceinline__ void unpack_ad(Raw w, float a[4], float d[4]) {
+    static_assert(DIFF, "binary16-base cells are difference-encoded");
+    const uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = base(q[i]);
+      d[i] = (float)((int)q[i] >> 16);
+    }
+  }
+};
 template <> struct CellLoad<int32_t> : CellLoad32<int32_t, int4> {};
 template <> struct CellLoad<uint32_t> : CellLoad32<uint32_t, uint4> {};
 

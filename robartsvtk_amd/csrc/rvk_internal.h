@@ -27,7 +27,7 @@ struct rvk_volume {
   int ncb[3] = {0, 0, 0};        // cell-brick grid: ceil((dims+2)/brick edge)
   size_t n_cells = 0;            // cells incl. brick padding (each 8 voxels)
   void* frames[RVK_MAX_FRAMES] = {};    // cell volumes, n_cells * 8 voxels
-  uint8_t diff[RVK_MAX_FRAMES] = {};    // 1: cells store x-pair differences (see Cells)
+  uint8_t enc[RVK_MAX_FRAMES] = {};     // cell encoding of each frame: kEnc* (see Cells, Half16)
   uint8_t* gradmag[RVK_MAX_FRAMES] = {};  // u8 cell volumes
   float gradmag_scale[RVK_MAX_FRAMES] = {};
   // Value range (min, max) of every brick of kRangeEdge^3 cells, for the DRR
@@ -93,29 +93,46 @@ int hip_fail(hipError_t e, const char* what);
 
 size_t dtype_size(int dtype);
 
+// Cell encodings of a frame: x-pairs stored as (a, b), as (a, b - a), or as
+// (a as binary16, b - a as int16) (Half16; 16-bit frames within +-2048).
+constexpr int kEncPlain = 0;
+constexpr int kEncDiff = 1;
+constexpr int kEncHalf = 2;
+
 // Calls f(Cells<T, L, D>{nullptr}) for a frame's voxel type, cell order and
 // encoding: float cells always store x-pair differences, 16-bit cells when
-// the frame's value range allows (`diff`), 8- and 32-bit integer cells never.
-template <int L, typename F> int dispatch_cells_t(int dtype, bool diff, F&& f) {
+// the frame's value range allows, 8- and 32-bit integer cells never.  With
+// HALF = false the binary16-base encoding is not dispatched (callers convert
+// such frames first, see frame_drop_half).
+template <int L, bool HALF, typename F> int dispatch_cells_t(int dtype, int enc, F&& f) {
   switch (dtype) {
     case RVK_U8: return f(Cells<uint8_t, L>{nullptr});
     case RVK_I8: return f(Cells<int8_t, L>{nullptr});
-    case RVK_U16: return diff ? f(Cells<uint16_t, L, true>{nullptr}) : f(Cells<uint16_t, L>{nullptr});
-    case RVK_I16: return diff ? f(Cells<int16_t, L, true>{nullptr}) : f(Cells<int16_t, L>{nullptr});
+    case RVK_U16:
+    case RVK_I16:
+      if (enc == kEncHalf) {
+        if constexpr (HALF) return f(Cells<Half16, L, true>{nullptr});
+        else return fail(RVK_ERR_INVALID_ARGUMENT, "binary16-base cells are not dispatched here");
+      }
+      if (dtype == RVK_U16) return enc ? f(Cells<uint16_t, L, true>{nullptr}) : f(Cells<uint16_t, L>{nullptr});
+      return enc ? f(Cells<int16_t, L, true>{nullptr}) : f(Cells<int16_t, L>{nullptr});
     case RVK_U32: return f(Cells<uint32_t, L>{nullptr});
     case RVK_I32: return f(Cells<int32_t, L>{nullptr});
     case RVK_F32: return f(Cells<float, L, true>{nullptr});
     default: return fail(RVK_ERR_INVALID_ARGUMENT, "unknown dtype %d", dtype);
   }
 }
-template <typename F> int dispatch_cells(int dtype, int layout, bool diff, F&& f) {
-  if (layout == kCellLayoutBrick4) return dispatch_cells_t<kCellLayoutBrick4>(dtype, diff, f);
-  return dispatch_cells_t<kCellLayoutPlain>(dtype, diff, f);
+template <bool HALF = true, typename F> int dispatch_cells(int dtype, int layout, int enc, F&& f) {
+  if (layout == kCellLayoutBrick4) return dispatch_cells_t<kCellLayoutBrick4, HALF>(dtype, enc, f);
+  return dispatch_cells_t<kCellLayoutPlain, HALF>(dtype, enc, f);
 }
 // The cells of one frame of a volume.
-template <typename F> int dispatch_frame(const rvk_volume* v, int frame, F&& f) {
-  return dispatch_cells(v->dtype, v->layout, v->diff[frame] != 0, f);
+template <bool HALF = true, typename F> int dispatch_frame(const rvk_volume* v, int frame, F&& f) {
+  return dispatch_cells<HALF>(v->dtype, v->layout, v->enc[frame], f);
 }
+// Re-encodes a binary16-base frame as plain differences (same values), on
+// stream s; a no-op for other frames.
+int frame_drop_half(const rvk_volume* v, int frame, hipStream_t s);
 VolGeom vol_geom(const rvk_volume* v);
 ViewGeom view_geom(const rvk_view* v);
 float min_spacing(const rvk_volume* v);

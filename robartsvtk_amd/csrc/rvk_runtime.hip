@@ -130,7 +130,7 @@ template <> struct CellVec<int16_t> { using T = uint4; };
 // DIFF stores each x-pair (a, b) as (a, b - a): exactly, as a 16-bit two's
 // complement difference (the frame's value range is below 32768) or, for
 // float, as the rounded difference the trilinear lerp computes from (a, b).
-template <typename VT, int L, bool DIFF = false>
+template <typename VT, int L, bool DIFF = false, bool HALF = false>
 __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst, int nx,
                                                             int ny, int nz, int ncbx, int ncby, long long n_cells) {
   using O = CellOrder<L>;
@@ -159,6 +159,11 @@ __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict
       if constexpr (sizeof(VT) == 2) v[q] = (VT)(uint16_t)((int)v[q] - (int)v[q - 1]);
       else v[q] = v[q] - v[q - 1];
     }
+  }
+  if constexpr (HALF && sizeof(VT) == 2) {
+    // bases in [-2048, 2048]: exact as binary16
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) v[q] = (VT)__builtin_bit_cast(uint16_t, (_Float16)(float)(int)v[q]);
   }
   VT* out = dst + c * 8;
   if constexpr (sizeof(VT) == 4) {
@@ -194,11 +199,11 @@ __global__ __launch_bounds__(256) void gradmag_kernel(CV vol, uint8_t* __restric
   out[e] = (uint8_t)fminf(rintf(q), 255.0f);
 }
 
-template <typename VT, int L, bool DIFF = false>
+template <typename VT, int L, bool DIFF = false, bool HALF = false>
 static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
   const long long total = (long long)v->n_cells;
   const int blocks = (int)((total + 255) / 256);
-  hipLaunchKernelGGL((cell_relayout_kernel<VT, L, DIFF>), dim3(blocks), dim3(256), 0, v->ctx->stream,
+  hipLaunchKernelGGL((cell_relayout_kernel<VT, L, DIFF, HALF>), dim3(blocks), dim3(256), 0, v->ctx->stream,
                      (const VT*)dsrc, (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
   RVK_HIP(hipGetLastError());
   return RVK_OK;
@@ -232,6 +237,27 @@ __global__ __launch_bounds__(256) void brick_range_kernel(const VT* __restrict__
       }
   const float w = hi > lo ? fmaxf(fabsf(lo), fabsf(hi)) * 0x1p-20f : 0.0f;
   out[b] = finite ? make_float2(lo - w, hi + w) : make_float2(-inf, inf);
+}
+
+// Binary16-base cells -> difference cells: each x-pair's base becomes the
+// 16-bit integer it holds (exact), the difference stays.
+__global__ __launch_bounds__(256) void half_to_diff_kernel(uint32_t* __restrict__ w, long long n_words) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_words) return;
+  const uint32_t q = w[i];
+  const int a = (int)(float)__builtin_bit_cast(_Float16, (uint16_t)(q & 0xffffu));
+  w[i] = (q & 0xffff0000u) | ((uint32_t)a & 0xffffu);
+}
+
+int frame_drop_half(const rvk_volume* cv, int frame, hipStream_t s) {
+  if (cv->enc[frame] != kEncHalf) return RVK_OK;
+  rvk_volume* v = const_cast<rvk_volume*>(cv);
+  const long long n = (long long)v->n_cells * 4;  // 8 x 16-bit per cell
+  hipLaunchKernelGGL(half_to_diff_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     (uint32_t*)v->frames[frame], n);
+  RVK_HIP(hipGetLastError());
+  v->enc[frame] = kEncDiff;
+  return RVK_OK;
 }
 
 // Value range of a 16-bit frame (decides its cell encoding).
@@ -512,20 +538,29 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
     dsrc = staging;
   }
   // float cells always store x-pair differences; 16-bit cells when the
-  // frame's values span less than 32768 (then every difference fits)
+  // frame's values span less than 32768 (then every difference fits), with
+  // binary16 bases when they all lie in [-2048, 2048]
   int rc = RVK_OK;
-  bool diff = v->dtype == RVK_F32;
+  int enc = v->dtype == RVK_F32 ? kEncDiff : kEncPlain;
   if (v->dtype == RVK_U16 || v->dtype == RVK_I16) {
     int lo = 0, hi = 0;
     rc = v->dtype == RVK_U16 ? frame_range<uint16_t>(v, dsrc, lo, hi) : frame_range<int16_t>(v, dsrc, lo, hi);
-    diff = rc == RVK_OK && hi - lo <= 32767;
+    if (rc == RVK_OK && hi - lo <= 32767) enc = (lo >= -2048 && hi <= 2048) ? kEncHalf : kEncDiff;
   }
   if (rc == RVK_OK) {
-    v->diff[frame] = diff;
-    rc = dispatch_cells(v->dtype, v->layout, diff, [&](auto tag) {
+    v->enc[frame] = (uint8_t)enc;
+    // dispatched on the source voxel type: binary16-base cells are written
+    // by the difference relayout of their 16-bit source
+    rc = dispatch_cells<false>(v->dtype, v->layout, enc == kEncHalf ? kEncDiff : enc, [&](auto tag) {
       using CV = decltype(tag);
       using VT = typename CV::value_type;
-      int r = launch_relayout<VT, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);
+      int r = RVK_OK;
+      if constexpr (sizeof(VT) == 2 && CV::kDiff) {
+        if (enc == kEncHalf) r = launch_relayout<VT, CV::kLog2, true, true>(v, dsrc, v->frames[frame]);
+        else r = launch_relayout<VT, CV::kLog2, true>(v, dsrc, v->frames[frame]);
+      } else {
+        r = launch_relayout<VT, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);
+      }
       if (r == RVK_OK) {
         const int nb = v->nrb[0] * v->nrb[1] * v->nrb[2];
         if (!v->ranges[frame]) RVK_HIP(hipMalloc((void**)&v->ranges[frame], sizeof(float2) * (size_t)nb));

@@ -98,6 +98,41 @@ def test_drr_wide_range_16bit_frames(gpu_ctx, dtype):
         _drr_check(gpu_ctx, vol, s.spacing, s.views[0], s.march, drr)
 
 
+@pytest.mark.parametrize("dtype", ["i16", "u16"])
+def test_binary16_base_frames(gpu_ctx, dtype):
+    """16-bit frames whose values all lie in [-2048, 2048] keep binary16
+    bases in their cells; they render exactly as the same values in float32
+    cells do, up to the end points of that interval and one past them (a
+    frame reaching 2049 keeps int16 bases)."""
+    s = configs.scene_c2(scale=0.125)
+    hu = s.volume.astype(np.float64)
+    t = (hu - hu.min()) / (hu.max() - hu.min())  # 0 (air) .. 1 (densest bone)
+    for top in (2048, 2049):
+        if dtype == "i16":  # stretched onto [-2048, top]
+            v = np.rint(-2048 + t * (top + 2048))
+            drr = dict(s.drr, slope=0.05 / (top + 2048), intercept=0.05 * 2048 / (top + 2048))
+        else:
+            v = np.rint(t * top)
+            drr = dict(s.drr, slope=0.05 / top, intercept=0.0)
+        vol = v.astype(np.int16 if dtype == "i16" else np.uint16)
+        assert int(vol.max()) == top
+        g = _drr_check(gpu_ctx, vol, s.spacing, s.views[0], s.march, drr)
+        vf = H.gpu_volume(gpu_ctx, vol.astype(np.float32), s.spacing)
+        gf = H.gpu_drr(gpu_ctx, vf, s.views[0], s.march, drr)
+        vf.close()
+        assert np.array_equal(g, gf), top
+    # a compositing caster on binary16-base cells: c1's u8 values as uint16
+    c1 = configs.scene_c1(scale=0.25)
+    march = dict(c1.march, term_T=1.0 / 64)
+    v8 = H.gpu_volume(gpu_ctx, c1.volume, c1.spacing)
+    v16 = H.gpu_volume(gpu_ctx, c1.volume.astype(np.uint16), c1.spacing)
+    a = H.gpu_1d(gpu_ctx, v8, c1.tf1d, c1.views[0], march)
+    b = H.gpu_1d(gpu_ctx, v16, c1.tf1d, c1.views[0], march)
+    v8.close()
+    v16.close()
+    assert np.array_equal(a, b) and a[..., 3].max() > 200
+
+
 @pytest.mark.parametrize("skip", [0, 1])
 @pytest.mark.parametrize("precise", [0, 1])
 @pytest.mark.parametrize("jitter", [0, 1])

@@ -188,6 +188,43 @@ def test_dual_mixed_cell_encodings(gpu_ctx):
     h2.close()
 
 
+def test_dual_binary16_base_component(gpu_ctx):
+    """Component 1 within [0, 2048] (binary16-base cells), component 2 up to
+    4095 (int16-difference cells): the dual caster re-encodes the first as
+    differences (same values) and reads both; a DRR plan captured on the
+    first volume before is not replayed on the re-encoded frame."""
+    import torch
+    s, view, v2, tf = _dual_scene()
+    v1 = (s.volume // 2).astype(np.uint16)
+    assert int(v1.max()) <= 2048 < int(v2.max())
+    tf = dict(tf, params=dict(tf["params"], intensity_multiplier=float(np.float32(2.0 / 4095.0))))
+    march = dict(s.march, term_T=1.0 / 32)
+    h1 = H.gpu_volume(gpu_ctx, v1, s.spacing)
+    h2 = H.gpu_volume(gpu_ctx, v2, s.spacing)
+    c2 = configs.scene_c2(scale=0.125)
+    mp = configs.march_params(c2.march)
+    dp = configs.drr_params(dict(c2.drr, slope=1e-5, intercept=0.0))
+    plan = abi.DrrPlan(gpu_ctx, h1, [view], mp, dp)
+    img = torch.zeros(plan.total, dtype=torch.float32, device="cuda")
+    plan.capture(img)
+    plan.render(img)
+    torch.cuda.synchronize()
+    before = img.clone()
+    g = H.gpu_dual(gpu_ctx, h1, h2, tf, view, march)
+    vd = configs.view_dict(view)
+    o32, _, _ = orc.cast_dual(v1, v2, s.spacing, tf, vd, march, "f32")
+    o64, _, _ = orc.cast_dual(v1, v2, s.spacing, tf, vd, dict(march, f32_control=1), "f64")
+    _check_rgba(g, o32, o64, "dual, binary16-base component")
+    assert (o32[..., 3] > 0).mean() > 0.05
+    img.zero_()
+    plan.render(img)
+    torch.cuda.synchronize()
+    assert torch.equal(img, before) and float(before.min()) < 0.9
+    plan.close()
+    h1.close()
+    h2.close()
+
+
 def test_siblings_identical_across_cell_layouts(gpu_ctx):
     s, view, v2, tf = _dual_scene()
     _keyhole(view, s.dims[0])

@@ -178,8 +178,9 @@ int rvk_copy_to_host(rvk_ctx* ctx, void* dst, const void* src, size_t bytes);
  * (VTK order) host or device memory, borrowed only for the duration of the
  * call.  Float cells store each x-pair of voxels as (a, b - a); so do 16-bit
  * cells when the frame's values span less than 32768 (checked on the device
- * at upload, which then waits for the check).  Renders are bit-identical in
- * either encoding. */
+ * at upload, which then waits for the check), with a stored as an IEEE
+ * binary16 when all values lie in [-2048, 2048].  Renders are bit-identical
+ * in every encoding. */
 int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[3], int dtype,
                       int n_frames, rvk_volume** out);
 int rvk_volume_set_frame(rvk_volume* vol, int frame, const void* src, int src_on_device);
@@ -241,7 +242,10 @@ int rvk_render_2d_inex(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk
  * (V/CUDA_vtkCudaDualImageVolumeMapper_renderAlgo.h:29-33): the two components
  * of the reference's 2-component image as two volumes of one size, type and
  * cell layout (frame applies to both).  tf's own params are not used; its
- * keyhole tables select the keyhole-TF variant. */
+ * keyhole tables select the keyhole-TF variant.  A component frame stored
+ * with binary16 bases is re-encoded in place, on ctx's stream, with int16
+ * bases (the same values): work on other streams that reads that frame must
+ * not overlap the call. */
 int rvk_render_dual(rvk_ctx* ctx, const rvk_volume* vol1, const rvk_volume* vol2, int frame, const rvk_tf2d* tf,
                     const rvk_dual_params* params, const rvk_view* view, const rvk_march_params* march,
                     uint8_t* out_rgba, const rvk_aux_out* aux);

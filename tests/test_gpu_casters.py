@@ -131,6 +131,18 @@ def test_binary16_base_frames(gpu_ctx, dtype):
     v8.close()
     v16.close()
     assert np.array_equal(a, b) and a[..., 3].max() > 200
+    # the 2D caster with the gradient-magnitude volume computed from them:
+    # c3's values halved (0..2047) against the same values in float32 cells
+    c3 = configs.scene_c3(scale=0.25)
+    half = c3.volume // 2
+    tf = dict(c3.tf2d, params=dict(c3.tf2d["params"], intensity_multiplier=float(np.float32(2.0 / 4095.0))))
+    outs = []
+    for vol in (half.astype(np.uint16), half.astype(np.float32)):
+        vh = H.gpu_volume(gpu_ctx, vol, c3.spacing)
+        vh.compute_gradmag(0, c3.tf2d["gradmag_max"])
+        outs.append(H.gpu_2d(gpu_ctx, vh, tf, c3.views[0], c3.march, use_gradmag=True))
+        vh.close()
+    assert np.array_equal(outs[0], outs[1]) and outs[0][..., 3].max() > 100
 
 
 @pytest.mark.parametrize("skip", [0, 1])

@@ -209,7 +209,8 @@ struct NoVisit {
 // same to the bit.  Runs pay where the rays of a wave skip together (many
 // views in one launch); a single view's launch is ordered for L2 locality,
 // which runs unbalance, and marches without them.
-template <typename CV, bool SKIP, bool PRECISE, bool VG = false, bool RUNS = false, typename F = NoVisit>
+template <typename CV, bool SKIP, bool PRECISE, bool VG = false, bool RUNS = false, bool KEY = true,
+          typename F = NoVisit>
 __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMarchArgs& a,
                                            const Ray& r, int x, int y, F&& visit = F()) {
   using FV = typename std::remove_reference<F>::type;
@@ -248,7 +249,8 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
   // a repeated alpha is the cheap hint of a constant brick
   float alpha_prev = __builtin_nanf("");
   while (t < tend) {
-    const bool in_key = (t >= kw0 && t < kw1);
+    // KEY = false: the view has no keyhole planes, so no ray has a window
+    const bool in_key = KEY && (t >= kw0 && t < kw1);
     const V3<float> q = sample_pos<PRECISE>(p, p0c, inc, t);
     V3<float> gr{0.0f, 0.0f, 0.0f};
     const float alpha = fmaf(cs, VG ? trilinear_vg<!PRECISE>(vol, g, q.x, q.y, q.z, gr)

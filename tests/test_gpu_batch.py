@@ -56,6 +56,55 @@ def test_batch_matches_single_view_renders(gpu_ctx, schedule):
     plan.close()
 
 
+def _with_keyhole(view, n):
+    key = [(1, 0, 0.01, -0.35 * n), (-1, 0, 0.01, 0.65 * n), (0, 1, 0.01, -0.3 * n), (0, -1, 0.01, 0.7 * n),
+           (0.01, 0, 1, -0.3 * n), (0.01, 0, -1, 0.6 * n)]
+    view.n_keyhole_planes = 6
+    view.keyhole_planes[:] = [float(x) for p in key for x in p]
+    return view
+
+
+def test_batch_views_with_keyhole_planes(gpu_ctx):
+    """A plan with keyhole planes on some views renders every view as the
+    single-view caster does (which specialises views without planes), also
+    after update_views adds or removes the planes of a captured plan."""
+    torch = _torch()
+    s = configs.scene_c4(scale=0.125, n_poses=6)
+    n = s.dims[0]
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    mp, dp = configs.march_params(s.march), configs.drr_params(s.drr)
+    plain = list(s.views)
+    keyed = [_with_keyhole(abi.View.from_buffer_copy(v), n) if i % 2 else v for i, v in enumerate(plain)]
+
+    def check(plan, views, out):
+        for v, view in enumerate(views):
+            single = torch.empty((view.height, view.width), dtype=torch.float32, device="cuda")
+            abi.render_drr(gpu_ctx, vol, view, mp, dp, single)
+            torch.cuda.synchronize()
+            assert torch.equal(plan.split(out)[v], single), v
+
+    plan = abi.DrrPlan(gpu_ctx, vol, plain, mp, dp)
+    out = torch.zeros(plan.total, dtype=torch.float32, device="cuda")
+    plan.capture(out)
+    plan.render(out)
+    torch.cuda.synchronize()
+    check(plan, plain, out)
+    base = out.clone()
+    plan.update_views(keyed)
+    out.zero_()
+    plan.render(out)
+    torch.cuda.synchronize()
+    check(plan, keyed, out)
+    assert not torch.equal(out, base)  # the keyholes changed some pixels
+    plan.update_views(plain)
+    out.zero_()
+    plan.render(out)
+    torch.cuda.synchronize()
+    assert torch.equal(out, base)
+    plan.close()
+    vol.close()
+
+
 def test_autotune_and_graph_replays(gpu_ctx):
     torch = _torch()
     s = configs.scene_c4(scale=0.125, n_poses=16)

@@ -29,14 +29,16 @@ struct DrrArgs {
   int tiles_x;
 };
 
-template <typename CV, bool SKIP, bool PRECISE, bool KEY>
+// KEY = false: the view has no keyhole planes; TERM = false: no termination
+// (the accumulated sum is never compared).
+template <typename CV, bool SKIP, bool PRECISE, bool KEY, bool TERM>
 __global__ __launch_bounds__(256) void drr_kernel(DrrArgs a) {
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int x, y;
   tile_pixel(tile, a.tiles_x, x, y);
   if (x >= a.v.W || y >= a.v.H) return;
   const Ray r = form_ray(a.g, a.v, a.step_len, x, y);
-  const float v = drr_march<CV, SKIP, PRECISE, false, false, KEY>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
+  const float v = drr_march<CV, SKIP, PRECISE, false, false, KEY, TERM>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
   const long idx = (long)y * a.v.W + x;
   if (a.m.output == RVK_DRR_RGBA8_REF) {
     // V/...DRR...cuh:163-166
@@ -433,18 +435,24 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
   return dispatch_frame(vol, frame, [&](auto tag) {
     using CV = decltype(tag);
     const bool pr = march->precise != 0, skip = march->skip_rule != 0;
+#define RVK_LDRR2(S, P, K)                                                                                 \
+  do {                                                                                                     \
+    if (drr->ref_termination)                                                                              \
+      hipLaunchKernelGGL((drr_kernel<CV, S, P, K, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);       \
+    else                                                                                                   \
+      hipLaunchKernelGGL((drr_kernel<CV, S, P, K, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);      \
+  } while (0)
 #define RVK_LDRR(S, P)                                                                                     \
   do {                                                                                                     \
-    if (view->n_keyhole_planes)                                                                            \
-      hipLaunchKernelGGL((drr_kernel<CV, S, P, true>), dim3(nblk), dim3(256), 0, ctx->stream, a);          \
-    else                                                                                                   \
-      hipLaunchKernelGGL((drr_kernel<CV, S, P, false>), dim3(nblk), dim3(256), 0, ctx->stream, a);         \
+    if (view->n_keyhole_planes) RVK_LDRR2(S, P, true);                                                     \
+    else RVK_LDRR2(S, P, false);                                                                           \
   } while (0)
     if (skip && pr) RVK_LDRR(true, true);
     else if (skip) RVK_LDRR(true, false);
     else if (pr) RVK_LDRR(false, true);
     else RVK_LDRR(false, false);
 #undef RVK_LDRR
+#undef RVK_LDRR2
     RVK_HIP(hipGetLastError());
     return (int)RVK_OK;
   });

@@ -210,7 +210,7 @@ struct NoVisit {
 // views in one launch); a single view's launch is ordered for L2 locality,
 // which runs unbalance, and marches without them.
 template <typename CV, bool SKIP, bool PRECISE, bool VG = false, bool RUNS = false, bool KEY = true,
-          typename F = NoVisit>
+          bool TERM = true, typename F = NoVisit>
 __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMarchArgs& a,
                                            const Ray& r, int x, int y, F&& visit = F()) {
   using FV = typename std::remove_reference<F>::type;
@@ -273,7 +273,7 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
         if (VG && sa == 0.0f) visit(q, ts, alpha, gr);
         // alpha - alpha * sa: alpha, or +0 for a back-stepped sample
         acc += (Acc)fmaf(-alpha, sa, alpha);
-        if (acc <= stop_at) break;
+        if (TERM && acc <= stop_at) break;
         // each further sample of a contributing run steps 1 (sa = sb = 0)
         if (FV::kFlatRuns && runs && repeat && sa == 0.0f) {
           if (flat_run(a, q, p0c, inc, w, cs, ci, alpha, t, t >= kw1 ? tend : flat_lim, acc, stop_at)) break;
@@ -301,7 +301,7 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
       if (alpha < 0.0f && !in_key) {
         acc += (Acc)alpha;
         if (VG) visit(q, ts, alpha, gr);
-        if (acc <= stop_at) break;
+        if (TERM && acc <= stop_at) break;
         if (FV::kFlatRuns && runs && repeat) {
           if (flat_run(a, q, p0c, inc, w, cs, ci, alpha, t, t >= kw1 ? tend : flat_lim, acc, stop_at)) break;
         }

@@ -78,7 +78,7 @@ __device__ __forceinline__ void item_pixel(int tiles16_x, int local, int lane, i
 // One pixel of one view.  With JAC the march also gathers trilinear gradients
 // and the pixel's pose Jacobian dL/dpose_i = A . dstart_i + B . dinc_i is
 // written next to the image (the registration step's single march).
-template <typename CV, bool SKIP, bool PRECISE, bool JAC>
+template <typename CV, bool SKIP, bool PRECISE, bool JAC, bool TERM>
 __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int lane) {
   const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
   const BatchView& bv = a.views[v];
@@ -88,14 +88,14 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
   const Ray r = form_ray(a.g, bv.v, a.step_len, x, y);
   const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
   if (!JAC) {
-    a.out[pi] = drr_march<CV, SKIP, PRECISE, false, true>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
+    a.out[pi] = drr_march<CV, SKIP, PRECISE, false, true, true, TERM>(cells_of<CV>(a.vol), a.g, a.m, r, x, y);
     return;
   }
   const float ds = sqrtf(r.inc.x * r.inc.x * a.g.sp[0] * a.g.sp[0] + r.inc.y * r.inc.y * a.g.sp[1] * a.g.sp[1] +
                          r.inc.z * r.inc.z * a.g.sp[2] * a.g.sp[2]);
   V3<float> A{0, 0, 0}, B{0, 0, 0};
   PoseVisit visit{ds * a.m.slope, ray_offset(a.m.jitter, x, y), A, B};
-  a.out[pi] = drr_march<CV, SKIP, true, true, true>(cells_of<CV>(a.vol), a.g, a.m, r, x, y, visit);
+  a.out[pi] = drr_march<CV, SKIP, true, true, true, true, TERM>(cells_of<CV>(a.vol), a.g, a.m, r, x, y, visit);
   const bool any = A.x != 0.0f || A.y != 0.0f || A.z != 0.0f || B.x != 0.0f || B.y != 0.0f || B.z != 0.0f;
 #pragma unroll 1
   for (int i = 0; i < 6; ++i) {
@@ -110,19 +110,19 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
 }
 
 // One wave per tile; workgroups XCD-remapped so neighbouring tiles share an L2.
-template <typename CV, bool SKIP, bool PRECISE, bool JAC>
+template <typename CV, bool SKIP, bool PRECISE, bool JAC, bool TERM>
 __global__ __launch_bounds__(256) void batch_grid_kernel(BatchArgs a) {
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int item = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (item >= a.n_items) return;
-  forward_item<CV, SKIP, PRECISE, JAC>(a, item, threadIdx.x & 63);
+  forward_item<CV, SKIP, PRECISE, JAC, TERM>(a, item, threadIdx.x & 63);
 }
 
 // Persistent: the tile range is cut into kNumXcd contiguous queues.  A wave
 // drains the queue of the XCD its workgroup was dispatched to (blockIdx mod
 // kNumXcd under round-robin dispatch), then the others in turn; every queue
 // is finite, so every wave reaches the end.
-template <typename CV, bool SKIP, bool PRECISE, bool JAC>
+template <typename CV, bool SKIP, bool PRECISE, bool JAC, bool TERM>
 __global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a) {
   const int lane = threadIdx.x & 63;
   const int home = blockIdx.x % kNumXcd;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a) {
       w = __builtin_amdgcn_readfirstlane(w);
       const int item = begin + w;
       if (item >= end) break;
-      forward_item<CV, SKIP, PRECISE, JAC>(a, item, lane);
+      forward_item<CV, SKIP, PRECISE, JAC, TERM>(a, item, lane);
     }
   }
 }
@@ -624,14 +624,26 @@ int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s, bool jac = false)
     if (sched == RVK_SCHED_PERSISTENT) {
       hipLaunchKernelGGL(reset_queues_kernel, dim3(1), dim3(64), 0, s, p->d_queues);
       const int blocks = std::max(1, std::min(p->n_cu * wps, (p->n_items + 3) / 4));
-#define RVK_LP(S, P, J) hipLaunchKernelGGL((batch_persistent_kernel<CV, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
+#define RVK_LP(S, P, J)                                                                                     \
+  do {                                                                                                      \
+    if (p->drr.ref_termination)                                                                             \
+      hipLaunchKernelGGL((batch_persistent_kernel<CV, S, P, J, true>), dim3(blocks), dim3(256), 0, s, a);    \
+    else                                                                                                    \
+      hipLaunchKernelGGL((batch_persistent_kernel<CV, S, P, J, false>), dim3(blocks), dim3(256), 0, s, a);   \
+  } while (0)
       if (jac) { if (skip) RVK_LP(true, true, true); else RVK_LP(false, true, true); }
       else if (skip) { if (pr) RVK_LP(true, true, false); else RVK_LP(true, false, false); }
       else { if (pr) RVK_LP(false, true, false); else RVK_LP(false, false, false); }
 #undef RVK_LP
     } else {
       const int blocks = (p->n_items + 3) / 4;
-#define RVK_LG(S, P, J) hipLaunchKernelGGL((batch_grid_kernel<CV, S, P, J>), dim3(blocks), dim3(256), 0, s, a)
+#define RVK_LG(S, P, J)                                                                                     \
+  do {                                                                                                      \
+    if (p->drr.ref_termination)                                                                             \
+      hipLaunchKernelGGL((batch_grid_kernel<CV, S, P, J, true>), dim3(blocks), dim3(256), 0, s, a);          \
+    else                                                                                                    \
+      hipLaunchKernelGGL((batch_grid_kernel<CV, S, P, J, false>), dim3(blocks), dim3(256), 0, s, a);         \
+  } while (0)
       if (jac) { if (skip) RVK_LG(true, true, true); else RVK_LG(false, true, true); }
       else if (skip) { if (pr) RVK_LG(true, true, false); else RVK_LG(true, false, false); }
       else { if (pr) RVK_LG(false, true, false); else RVK_LG(false, false, false); }

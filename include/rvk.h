@@ -192,7 +192,8 @@ int rvk_volume_destroy(rvk_volume* vol);
  * keeps a float cudaArray whose tiling the driver picks).  Each trilinear
  * 2x2x2 footprint is stored whole; RVK_CELLS_PLAIN orders the cells x-fastest,
  * RVK_CELLS_BRICK4 groups them in 4x4x4 bricks.  The default is BRICK4 when
- * the cells take more than 1 GiB, PLAIN below; set_layout must come before
+ * the cells take more than 1 GiB (or PLAIN would exceed its limit of 2^24
+ * cell rows, (ny + 2)(nz + 2)), PLAIN below; set_layout must come before
  * the first rvk_volume_set_frame.  Renders give identical results in both. */
 enum { RVK_CELLS_PLAIN = 0, RVK_CELLS_BRICK4 = 1 };
 int rvk_volume_set_layout(rvk_volume* vol, int layout);

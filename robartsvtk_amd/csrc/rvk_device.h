@@ -22,6 +22,7 @@ namespace rvk {
 struct VolGeom {
   int n[3];        // dims
   uint32_t cys, czs;  // cell-brick row / slab strides in cells
+  uint32_t crows;     // cell-brick rows per slab (czs = crows * cys)
   float srecip[3]; // 0.5/spacing  (V/vtkCudaVolumeInformationHandler.cxx:91-93)
   float sp[3];     // spacing
   float bounds[6]; // [0, dim-1] per axis
@@ -104,7 +105,13 @@ template <int LOG2> struct CellOrder {
     return __umul24((uint32_t)(c >> LOG2), czs) + (uint32_t)((c & (kEdge - 1)) << (2 * LOG2));
   }
   static __device__ __forceinline__ uint32_t index(const VolGeom& g, int cx, int cy, int cz) {
-    return x(cx) + y(cy, g.cys) + z(cz, g.czs);
+    if constexpr (LOG2 == 0) {
+      // (cz * rows + cy) * cys + cx: two 24-bit multiply-adds (the row count
+      // of a volume, (ny + 2)(nz + 2), is below 2^24 like the slab stride)
+      return __umul24(__umul24((uint32_t)cz, g.crows) + (uint32_t)cy, g.cys) + (uint32_t)cx;
+    } else {
+      return x(cx) + y(cy, g.cys) + z(cz, g.czs);
+    }
   }
 };
 constexpr int kCellLayoutPlain = 0;

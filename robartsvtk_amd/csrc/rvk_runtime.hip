@@ -62,6 +62,7 @@ VolGeom vol_geom(const rvk_volume* v) {
   const uint32_t bc = 1u << (3 * v->layout);  // cells per brick
   g.cys = bc * (uint32_t)v->ncb[0];
   g.czs = bc * (uint32_t)v->ncb[0] * (uint32_t)v->ncb[1];
+  g.crows = (uint32_t)v->ncb[1];
   return g;
 }
 
@@ -316,7 +317,8 @@ static int launch_gradmag(rvk_volume* v, int frame, float inv_scale, uint8_t* li
 }
 
 // Cell grid and count of a volume in its cell order; fails when the cell
-// addressing limits (slab < 2^24 cells, < 2^32 cells) are exceeded.
+// addressing limits (slab < 2^24 cells, plain order: rows < 2^24, < 2^32
+// cells) are exceeded.
 static int size_cells(rvk_volume* v) {
   const int edge = 1 << v->layout;
   for (int a = 0; a < 3; ++a) v->ncb[a] = (v->dims[a] + 2 + edge - 1) / edge;
@@ -326,6 +328,9 @@ static int size_cells(rvk_volume* v) {
   const size_t slab = bc * v->ncb[0] * v->ncb[1];
   if (slab >= ((size_t)1 << 24))  // cell addressing multiplies with 24-bit operands
     return fail(RVK_ERR_INVALID_ARGUMENT, "volume x/y extent too large: %zu cells per z slab, limit 2^24 - 1", slab);
+  if (v->layout == kCellLayoutPlain && (size_t)v->ncb[1] * v->ncb[2] >= ((size_t)1 << 24))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "volume y/z extent too large for x-fastest cells: %zu cell rows, limit 2^24 - 1",
+                (size_t)v->ncb[1] * v->ncb[2]);
   if (v->n_cells > (size_t)0xffffffffu)
     return fail(RVK_ERR_INVALID_ARGUMENT, "volume too large: %zu cells exceed 32-bit cell indices", v->n_cells);
   return RVK_OK;
@@ -336,7 +341,8 @@ static int size_cells(rvk_volume* v) {
 // the caches), plain x-fastest order below.
 static int default_layout(const int32_t dims[3], int dtype) {
   const double bytes = 8.0 * (double)dtype_size(dtype) * (dims[0] + 2.0) * (dims[1] + 2.0) * (dims[2] + 2.0);
-  return bytes > (double)(1ull << 30) ? kCellLayoutBrick4 : kCellLayoutPlain;
+  const double rows = (dims[1] + 2.0) * (dims[2] + 2.0);
+  return (bytes > (double)(1ull << 30) || rows >= (double)(1 << 24)) ? kCellLayoutBrick4 : kCellLayoutPlain;
 }
 
 }  // namespace rvk

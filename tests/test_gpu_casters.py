@@ -525,8 +525,8 @@ def test_cell_layout_rules(gpu_ctx):
         small.set_layout(abi.CELLS_BRICK4)           # fixed once a frame has data
     # x-fastest cells address (nz + 2)(ny + 2) cell rows with 24-bit
     # multiplies: a thin volume with more rows defaults to bricks
-    thin = abi.Volume(gpu_ctx, (1, 4096, 4096), (1.0, 1.0, 1.0), np.uint8)
-    assert thin.layout() == abi.CELLS_BRICK4          # 50 MB of cells, 4098^2 rows
+    thin = abi.Volume(gpu_ctx, (2, 4096, 4096), (1.0, 1.0, 1.0), np.uint8)
+    assert thin.layout() == abi.CELLS_BRICK4          # 0.5 GB of cells, 4098^2 rows
     with pytest.raises(abi.RvkError):
         thin.set_layout(abi.CELLS_PLAIN)
     thin.close()

@@ -313,7 +313,6 @@ __device__ __forceinline__ float load_voxel(Cells<T, L, D> v, const VolGeom& g, 
 struct CellPos {
   uint32_t cell;
   float ax, ay, az;
-  bool xs, ys, zs;  // the two corners along the axis are distinct voxels
 };
 // CLAMP = false when the caller has checked that c lies in [0, n + 1].
 template <int L, bool CLAMP = true>
@@ -327,9 +326,6 @@ __device__ __forceinline__ CellPos cell_pos(const VolGeom& g, float cx, float cy
   c.az = __builtin_amdgcn_fractf(z);
   const int ix = (int)x, iy = (int)y, iz = (int)z;
   c.cell = CellOrder<L>::index(g, ix, iy, iz);
-  c.xs = ix >= 1 && ix < g.n[0];
-  c.ys = iy >= 1 && iy < g.n[1];
-  c.zs = iz >= 1 && iz < g.n[2];
   return c;
 }
 
@@ -354,8 +350,10 @@ __device__ __forceinline__ float trilinear(Cells<T, L, D> vol, const VolGeom& g,
 // computed with exactly the operations of trilinear(), so branch decisions
 // made on it match the plain caster.
 // d/dx = lerp_z(lerp_y(v100-v000, v110-v010), lerp_y(v101-v001, v111-v011)),
-// d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0; axes whose two corners
-// clamp to one voxel get a zero derivative.
+// d/dy = lerp_z(c10-c00, c11-c01), d/dz = c1-c0.  Along an axis whose two
+// corners clamp to one voxel (the edge cells 0, n and n + 1, where every
+// clamped coordinate also lands) the corners are equal, so these
+// differences, and the derivative, are exactly +0 with no test.
 template <bool CLAMP = true, typename T, int L, bool D>
 __device__ __forceinline__ float trilinear_vg(Cells<T, L, D> vol, const VolGeom& g, float cx, float cy, float cz,
                                               V3<float>& gr) {
@@ -366,9 +364,9 @@ __device__ __forceinline__ float trilinear_vg(Cells<T, L, D> vol, const VolGeom&
   const float c01 = fmaf(c.ax, d[2], a[2]), c11 = fmaf(c.ax, d[3], a[3]);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
   const float dx0 = lerp(d[0], d[1], c.ay), dx1 = lerp(d[2], d[3], c.ay);
-  gr.x = c.xs ? lerp(dx0, dx1, c.az) : 0.0f;
-  gr.y = c.ys ? lerp(c10 - c00, c11 - c01, c.az) : 0.0f;
-  gr.z = c.zs ? c1 - c0 : 0.0f;
+  gr.x = lerp(dx0, dx1, c.az);
+  gr.y = lerp(c10 - c00, c11 - c01, c.az);
+  gr.z = c1 - c0;
   return lerp(c0, c1, c.az);
 }
 

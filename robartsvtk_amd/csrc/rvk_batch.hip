@@ -45,19 +45,26 @@ struct BatchArgs {
   unsigned* queues;       // kNumXcd work-queue heads (persistent schedule)
 };
 
-// Accumulates A = sum ds*slope*grad mu and B = sum (j+t) ds*slope*grad mu over
-// the accumulated samples of one ray (see the pose adjoint below).
+// Accumulates S = sum grad mu and T = sum (j+t) grad mu over the accumulated
+// samples of one ray; A = ds*slope*S and B = ds*slope*T (see the pose adjoint
+// below) are formed once per ray by pose_sums().
 struct PoseVisit {
   static constexpr bool kFlatRuns = true;  // a zero gradient adds nothing
-  float ncs, j;
-  V3<float>& A;
-  V3<float>& B;
+  float j;
+  V3<float>& S;
+  V3<float>& T;
   __device__ __forceinline__ void operator()(const V3<float>&, float t, float, const V3<float>& gr) const {
     const float w = j + t;
-    A.x += ncs * gr.x; A.y += ncs * gr.y; A.z += ncs * gr.z;
-    B.x += w * ncs * gr.x; B.y += w * ncs * gr.y; B.z += w * ncs * gr.z;
+    S.x += gr.x; S.y += gr.y; S.z += gr.z;
+    T.x = fmaf(w, gr.x, T.x); T.y = fmaf(w, gr.y, T.y); T.z = fmaf(w, gr.z, T.z);
   }
 };
+// Scales the sums by ncs = ds*slope; called only for rays that accumulated
+// a sample (a ray that misses the volume can have a non-finite ds).
+__device__ __forceinline__ void pose_sums(float ncs, V3<float>& A, V3<float>& B) {
+  A.x *= ncs; A.y *= ncs; A.z *= ncs;
+  B.x *= ncs; B.y *= ncs; B.z *= ncs;
+}
 
 __device__ __forceinline__ int find_view(const int* __restrict__ ib, int n, int item) {
   int lo = 0, hi = n;  // ib[lo] <= item < ib[hi]
@@ -94,9 +101,10 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
   const float ds = sqrtf(r.inc.x * r.inc.x * a.g.sp[0] * a.g.sp[0] + r.inc.y * r.inc.y * a.g.sp[1] * a.g.sp[1] +
                          r.inc.z * r.inc.z * a.g.sp[2] * a.g.sp[2]);
   V3<float> A{0, 0, 0}, B{0, 0, 0};
-  PoseVisit visit{ds * a.m.slope, ray_offset(a.m.jitter, x, y), A, B};
+  PoseVisit visit{ray_offset(a.m.jitter, x, y), A, B};
   a.out[pi] = drr_march<CV, SKIP, true, true, true, true, TERM>(cells_of<CV>(a.vol), a.g, a.m, r, x, y, visit);
   const bool any = A.x != 0.0f || A.y != 0.0f || A.z != 0.0f || B.x != 0.0f || B.y != 0.0f || B.z != 0.0f;
+  if (any) pose_sums(ds * a.m.slope, A, B);
 #pragma unroll 1
   for (int i = 0; i < 6; ++i) {
     float jv = 0.0f;
@@ -279,13 +287,14 @@ __global__ __launch_bounds__(256) void adjoint_pose_kernel(AdjArgs a) {
                            r.inc.z * r.inc.z * g.sp[2] * g.sp[2]);
     V3<float> A{0, 0, 0}, B{0, 0, 0};
     const float jt = ray_offset(a.b.m.jitter, x, y);
-    PoseVisit visit{ds * a.b.m.slope, jt, A, B};
+    PoseVisit visit{jt, A, B};
     const float I = drr_march<CV, SKIP, true, true, true>(cells_of<CV>(a.b.vol), g, a.b.m, r, x, y, visit);
     const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
     const float dLdI = dloss_at(a.dldi, a.stats, a.img, a.tgt, v, pi);
     // dI/dL = -I for exp(-L) output, 1 for L output
     const float coef = (a.b.m.output == RVK_DRR_F32_EXP_NEG_L) ? -dLdI * I : dLdI;
     if (coef != 0.0f && (A.x != 0.0f || A.y != 0.0f || A.z != 0.0f)) {
+      pose_sums(ds * a.b.m.slope, A, B);
 #pragma unroll 1
       for (int i = 0; i < 6; ++i) {
         const RayTangent tg = form_ray_tangent(g, bv.v, bv.dm + 16 * i, a.b.step_len, x, y);

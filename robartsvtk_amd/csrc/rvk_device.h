@@ -288,6 +288,41 @@ template <> struct CellLoad<Half16> {
     }
   }
 };
+// When, in addition, every x-pair difference of the frame is an integer that
+// binary16 holds exactly (|b - a| <= 2048, or even up to 4096), the
+// difference is stored as binary16 too, and the first lerp level is one
+// v_fma_mix_f32 reading both halves of the word.  Same values again.
+struct HalfPair {
+  uint16_t bits;
+  __device__ __forceinline__ operator float() const { return (float)__builtin_bit_cast(_Float16, bits); }
+};
+template <> struct CellLoad<HalfPair> {
+  using Raw = uint4;
+  static __device__ __forceinline__ Raw raw(const HalfPair* __restrict__ d, uint32_t c) {
+    return *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ float lo(uint32_t q) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(q & 0xffffu));
+  }
+  static __device__ __forceinline__ float hi(uint32_t q) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(q >> 16)); }
+  static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
+    const uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[2 * i] = lo(q[i]);
+      v[2 * i + 1] = v[2 * i] + hi(q[i]);
+    }
+  }
+  template <bool DIFF> static __device__ __forceinline__ void unpack_ad(Raw w, float a[4], float d[4]) {
+    static_assert(DIFF, "binary16-pair cells are difference-encoded");
+    const uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = lo(q[i]);
+      d[i] = hi(q[i]);
+    }
+  }
+};
 template <> struct CellLoad<int32_t> : CellLoad32<int32_t, int4> {};
 template <> struct CellLoad<uint32_t> : CellLoad32<uint32_t, uint4> {};
 

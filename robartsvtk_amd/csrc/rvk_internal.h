@@ -94,15 +94,17 @@ int hip_fail(hipError_t e, const char* what);
 size_t dtype_size(int dtype);
 
 // Cell encodings of a frame: x-pairs stored as (a, b), as (a, b - a), or as
-// (a as binary16, b - a as int16) (Half16; 16-bit frames within +-2048).
+// (a as binary16, b - a as int16) (Half16; 16-bit frames within +-2048), or
+// with both as binary16 (HalfPair; also every difference exact in binary16).
 constexpr int kEncPlain = 0;
 constexpr int kEncDiff = 1;
 constexpr int kEncHalf = 2;
+constexpr int kEncHalfPair = 3;
 
 // Calls f(Cells<T, L, D>{nullptr}) for a frame's voxel type, cell order and
 // encoding: float cells always store x-pair differences, 16-bit cells when
 // the frame's value range allows, 8- and 32-bit integer cells never.  With
-// HALF = false the binary16-base encoding is not dispatched (callers convert
+// HALF = false the binary16 encodings are not dispatched (callers convert
 // such frames first, see frame_drop_half).
 template <int L, bool HALF, typename F> int dispatch_cells_t(int dtype, int enc, F&& f) {
   switch (dtype) {
@@ -110,9 +112,13 @@ template <int L, bool HALF, typename F> int dispatch_cells_t(int dtype, int enc,
     case RVK_I8: return f(Cells<int8_t, L>{nullptr});
     case RVK_U16:
     case RVK_I16:
-      if (enc == kEncHalf) {
-        if constexpr (HALF) return f(Cells<Half16, L, true>{nullptr});
-        else return fail(RVK_ERR_INVALID_ARGUMENT, "binary16-base cells are not dispatched here");
+      if (enc == kEncHalf || enc == kEncHalfPair) {
+        if constexpr (HALF) {
+          if (enc == kEncHalfPair) return f(Cells<HalfPair, L, true>{nullptr});
+          return f(Cells<Half16, L, true>{nullptr});
+        } else {
+          return fail(RVK_ERR_INVALID_ARGUMENT, "binary16 cells are not dispatched here");
+        }
       }
       if (dtype == RVK_U16) return enc ? f(Cells<uint16_t, L, true>{nullptr}) : f(Cells<uint16_t, L>{nullptr});
       return enc ? f(Cells<int16_t, L, true>{nullptr}) : f(Cells<int16_t, L>{nullptr});
@@ -130,8 +136,8 @@ template <bool HALF = true, typename F> int dispatch_cells(int dtype, int layout
 template <bool HALF = true, typename F> int dispatch_frame(const rvk_volume* v, int frame, F&& f) {
   return dispatch_cells<HALF>(v->dtype, v->layout, v->enc[frame], f);
 }
-// Re-encodes a binary16-base frame as plain differences (same values), on
-// stream s; a no-op for other frames.
+// Re-encodes a binary16 frame (Half16 or HalfPair cells) as int16
+// difference cells (same values), on stream s; a no-op for other frames.
 int frame_drop_half(const rvk_volume* v, int frame, hipStream_t s);
 VolGeom vol_geom(const rvk_volume* v);
 ViewGeom view_geom(const rvk_view* v);

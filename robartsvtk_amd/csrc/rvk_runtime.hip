@@ -131,7 +131,9 @@ template <> struct CellVec<int16_t> { using T = uint4; };
 // DIFF stores each x-pair (a, b) as (a, b - a): exactly, as a 16-bit two's
 // complement difference (the frame's value range is below 32768) or, for
 // float, as the rounded difference the trilinear lerp computes from (a, b).
-template <typename VT, int L, bool DIFF = false, bool HALF = false>
+// HALF (16-bit DIFF cells): 1 stores the bases as binary16, 2 the
+// differences too (both exact for the frames given those encodings).
+template <typename VT, int L, bool DIFF = false, int HALF = 0>
 __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst, int nx,
                                                             int ny, int nz, int ncbx, int ncby, long long n_cells) {
   using O = CellOrder<L>;
@@ -161,10 +163,15 @@ __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict
       else v[q] = v[q] - v[q - 1];
     }
   }
-  if constexpr (HALF && sizeof(VT) == 2) {
+  if constexpr (HALF > 0 && sizeof(VT) == 2) {
     // bases in [-2048, 2048]: exact as binary16
 #pragma unroll
     for (int q = 0; q < 8; q += 2) v[q] = (VT)__builtin_bit_cast(uint16_t, (_Float16)(float)(int)v[q]);
+    if constexpr (HALF > 1) {
+#pragma unroll
+      for (int q = 1; q < 8; q += 2)
+        v[q] = (VT)__builtin_bit_cast(uint16_t, (_Float16)(float)(int)(int16_t)(uint16_t)v[q]);
+    }
   }
   VT* out = dst + c * 8;
   if constexpr (sizeof(VT) == 4) {
@@ -200,7 +207,7 @@ __global__ __launch_bounds__(256) void gradmag_kernel(CV vol, uint8_t* __restric
   out[e] = (uint8_t)fminf(rintf(q), 255.0f);
 }
 
-template <typename VT, int L, bool DIFF = false, bool HALF = false>
+template <typename VT, int L, bool DIFF = false, int HALF = 0>
 static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
   const long long total = (long long)v->n_cells;
   const int blocks = (int)((total + 255) / 256);
@@ -240,61 +247,75 @@ __global__ __launch_bounds__(256) void brick_range_kernel(const VT* __restrict__
   out[b] = finite ? make_float2(lo - w, hi + w) : make_float2(-inf, inf);
 }
 
-// Binary16-base cells -> difference cells: each x-pair's base becomes the
-// 16-bit integer it holds (exact), the difference stays.
-__global__ __launch_bounds__(256) void half_to_diff_kernel(uint32_t* __restrict__ w, long long n_words) {
+// Binary16 cells -> difference cells: each x-pair's base (and, for
+// HalfPair cells, its difference) becomes the 16-bit integer it holds
+// (exact); an int16 difference stays.
+__global__ __launch_bounds__(256) void half_to_diff_kernel(uint32_t* __restrict__ w, long long n_words, int pair) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_words) return;
   const uint32_t q = w[i];
   const int a = (int)(float)__builtin_bit_cast(_Float16, (uint16_t)(q & 0xffffu));
-  w[i] = (q & 0xffff0000u) | ((uint32_t)a & 0xffffu);
+  const uint32_t d = pair ? (uint32_t)(int)(float)__builtin_bit_cast(_Float16, (uint16_t)(q >> 16)) << 16
+                          : q & 0xffff0000u;
+  w[i] = d | ((uint32_t)a & 0xffffu);
 }
 
 int frame_drop_half(const rvk_volume* cv, int frame, hipStream_t s) {
-  if (cv->enc[frame] != kEncHalf) return RVK_OK;
+  if (cv->enc[frame] != kEncHalf && cv->enc[frame] != kEncHalfPair) return RVK_OK;
   rvk_volume* v = const_cast<rvk_volume*>(cv);
   const long long n = (long long)v->n_cells * 4;  // 8 x 16-bit per cell
   hipLaunchKernelGGL(half_to_diff_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     (uint32_t*)v->frames[frame], n);
+                     (uint32_t*)v->frames[frame], n, (int)(cv->enc[frame] == kEncHalfPair));
   RVK_HIP(hipGetLastError());
   v->enc[frame] = kEncDiff;
   return RVK_OK;
 }
 
-// Value range of a 16-bit frame (decides its cell encoding).
+// Value range of a 16-bit frame and whether some x-neighbour difference is
+// not exact in binary16 (|d| > 2048 and odd; the flag is used only for
+// frames within [-2048, 2048], whose differences are at most 4096): these
+// decide the frame's cell encoding.
 __global__ __launch_bounds__(64) void range_init_kernel(int* out) {
   if (threadIdx.x == 0) {
     out[0] = 0x7fffffff;
     out[1] = -0x7fffffff - 1;
+    out[2] = 0;
   }
 }
 template <typename VT>
-__global__ __launch_bounds__(256) void range_kernel(const VT* __restrict__ src, long long n, int* out) {
-  int lo = 0x7fffffff, hi = -0x7fffffff - 1;
+__global__ __launch_bounds__(256) void range_kernel(const VT* __restrict__ src, long long n, int nx, int* out) {
+  int lo = 0x7fffffff, hi = -0x7fffffff - 1, odd = 0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const int x = (int)src[i];
     lo = min(lo, x);
     hi = max(hi, x);
+    if ((int)(i % nx) != nx - 1) {
+      const int d = abs((int)src[i + 1] - x);
+      odd |= (int)(d > 2048 && (d & 1));
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     lo = min(lo, __shfl_xor(lo, o, 64));
     hi = max(hi, __shfl_xor(hi, o, 64));
+    odd |= __shfl_xor(odd, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     atomicMin(out, lo);
     atomicMax(out + 1, hi);
+    if (odd) atomicMax(out + 2, 1);
   }
 }
 
 template <typename VT>
-static int frame_range(rvk_volume* v, const void* dsrc, int& lo, int& hi) {
+static int frame_range(rvk_volume* v, const void* dsrc, int& lo, int& hi, bool& odd_diff) {
   const long long n = (long long)v->dims[0] * v->dims[1] * v->dims[2];
   int* d = nullptr;
-  RVK_HIP(hipMallocAsync((void**)&d, 2 * sizeof(int), v->ctx->stream));
+  RVK_HIP(hipMallocAsync((void**)&d, 3 * sizeof(int), v->ctx->stream));
   hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(64), 0, v->ctx->stream, d);
   const int blocks = (int)std::min<long long>(4096, (n + 255) / 256);
-  hipLaunchKernelGGL((range_kernel<VT>), dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc, n, d);
-  int h[2] = {0, 0};
+  hipLaunchKernelGGL((range_kernel<VT>), dim3(blocks), dim3(256), 0, v->ctx->stream, (const VT*)dsrc, n, v->dims[0],
+                     d);
+  int h[3] = {0, 0, 0};
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, v->ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(v->ctx->stream);
@@ -302,6 +323,7 @@ static int frame_range(rvk_volume* v, const void* dsrc, int& lo, int& hi) {
   if (e != hipSuccess) return hip_fail(e, "frame value range");
   lo = h[0];
   hi = h[1];
+  odd_diff = h[2] != 0;
   return RVK_OK;
 }
 
@@ -545,24 +567,32 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
   }
   // float cells always store x-pair differences; 16-bit cells when the
   // frame's values span less than 32768 (then every difference fits), with
-  // binary16 bases when they all lie in [-2048, 2048]
+  // binary16 bases when they all lie in [-2048, 2048], and binary16
+  // differences when these are exact too
   int rc = RVK_OK;
   int enc = v->dtype == RVK_F32 ? kEncDiff : kEncPlain;
   if (v->dtype == RVK_U16 || v->dtype == RVK_I16) {
     int lo = 0, hi = 0;
-    rc = v->dtype == RVK_U16 ? frame_range<uint16_t>(v, dsrc, lo, hi) : frame_range<int16_t>(v, dsrc, lo, hi);
-    if (rc == RVK_OK && hi - lo <= 32767) enc = (lo >= -2048 && hi <= 2048) ? kEncHalf : kEncDiff;
+    bool odd = false;
+    rc = v->dtype == RVK_U16 ? frame_range<uint16_t>(v, dsrc, lo, hi, odd)
+                             : frame_range<int16_t>(v, dsrc, lo, hi, odd);
+    if (rc == RVK_OK && hi - lo <= 32767) {
+      enc = kEncDiff;
+      if (lo >= -2048 && hi <= 2048) enc = odd ? kEncHalf : kEncHalfPair;
+    }
   }
   if (rc == RVK_OK) {
     v->enc[frame] = (uint8_t)enc;
-    // dispatched on the source voxel type: binary16-base cells are written
-    // by the difference relayout of their 16-bit source
-    rc = dispatch_cells<false>(v->dtype, v->layout, enc == kEncHalf ? kEncDiff : enc, [&](auto tag) {
+    // dispatched on the source voxel type: binary16 cells are written by the
+    // difference relayout of their 16-bit source
+    const bool half = enc == kEncHalf || enc == kEncHalfPair;
+    rc = dispatch_cells<false>(v->dtype, v->layout, half ? kEncDiff : enc, [&](auto tag) {
       using CV = decltype(tag);
       using VT = typename CV::value_type;
       int r = RVK_OK;
       if constexpr (sizeof(VT) == 2 && CV::kDiff) {
-        if (enc == kEncHalf) r = launch_relayout<VT, CV::kLog2, true, true>(v, dsrc, v->frames[frame]);
+        if (enc == kEncHalfPair) r = launch_relayout<VT, CV::kLog2, true, 2>(v, dsrc, v->frames[frame]);
+        else if (enc == kEncHalf) r = launch_relayout<VT, CV::kLog2, true, 1>(v, dsrc, v->frames[frame]);
         else r = launch_relayout<VT, CV::kLog2, true>(v, dsrc, v->frames[frame]);
       } else {
         r = launch_relayout<VT, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);

@@ -101,26 +101,40 @@ def test_drr_wide_range_16bit_frames(gpu_ctx, dtype):
 @pytest.mark.parametrize("dtype", ["i16", "u16"])
 def test_binary16_base_frames(gpu_ctx, dtype):
     """16-bit frames whose values all lie in [-2048, 2048] keep binary16
-    bases in their cells; they render exactly as the same values in float32
-    cells do, up to the end points of that interval and one past them (a
-    frame reaching 2049 keeps int16 bases)."""
+    bases in their cells, and binary16 x-pair differences too when every
+    difference is exact in binary16 (|d| <= 2048, or even up to 4096); they
+    render exactly as the same values in float32 cells do, up to the end
+    points of that interval and one past them (a frame reaching 2049 keeps
+    int16 bases).  Cases: "pair" (values even: differences up to 4096, all
+    exact), "base" (i16 only: one x-pair (-2048, 2047) planted in the middle,
+    difference 4095, which binary16 would round), "int16" (top 2049)."""
     s = configs.scene_c2(scale=0.125)
     hu = s.volume.astype(np.float64)
     t = (hu - hu.min()) / (hu.max() - hu.min())  # 0 (air) .. 1 (densest bone)
-    for top in (2048, 2049):
+    for case in ("pair", "base", "int16") if dtype == "i16" else ("pair", "int16"):
+        top = 2049 if case == "int16" else 2048
         if dtype == "i16":  # stretched onto [-2048, top]
             v = np.rint(-2048 + t * (top + 2048))
             drr = dict(s.drr, slope=0.05 / (top + 2048), intercept=0.05 * 2048 / (top + 2048))
         else:
             v = np.rint(t * top)
             drr = dict(s.drr, slope=0.05 / top, intercept=0.0)
+        if case != "int16":
+            v = 2 * np.floor(v / 2)
+        if case == "base":
+            z, y, x = (n // 2 for n in v.shape)
+            v[z, y, x : x + 2] = (-2048, 2047)
         vol = v.astype(np.int16 if dtype == "i16" else np.uint16)
-        assert int(vol.max()) == top
+        assert int(vol.max()) == top and int(vol.min()) == (-2048 if dtype == "i16" else 0)
+        d = np.abs(np.diff(vol.astype(np.int32), axis=2))
+        if case != "int16":
+            assert ((d > 2048) & (d % 2 == 1)).any() == (case == "base")
+        assert d.max() > 2048 or dtype == "u16"
         g = _drr_check(gpu_ctx, vol, s.spacing, s.views[0], s.march, drr)
         vf = H.gpu_volume(gpu_ctx, vol.astype(np.float32), s.spacing)
         gf = H.gpu_drr(gpu_ctx, vf, s.views[0], s.march, drr)
         vf.close()
-        assert np.array_equal(g, gf), top
+        assert np.array_equal(g, gf), case
     # a compositing caster on binary16-base cells: c1's u8 values as uint16
     c1 = configs.scene_c1(scale=0.25)
     march = dict(c1.march, term_T=1.0 / 64)

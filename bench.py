@@ -297,8 +297,8 @@ def main() -> int:
                    "n_samples": r["n_samples"], "voxel_bytes": 2},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("drr_kernel<Cells<Half16,0,true>,true,true,false,false>"),
-                     "kernel": "drr_kernel<Cells<Half16,0,true>,true,true,false,false>", "kernel_ms": round(r["kernel_ms"], 4),
+                     "traffic": measured_traffic("drr_kernel<Cells<HalfPair,0,true>,true,true,false,false>"),
+                     "kernel": "drr_kernel<Cells<HalfPair,0,true>,true,true,false,false>", "kernel_ms": round(r["kernel_ms"], 4),
                      "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_casters:

@@ -179,8 +179,9 @@ int rvk_copy_to_host(rvk_ctx* ctx, void* dst, const void* src, size_t bytes);
  * call.  Float cells store each x-pair of voxels as (a, b - a); so do 16-bit
  * cells when the frame's values span less than 32768 (checked on the device
  * at upload, which then waits for the check), with a stored as an IEEE
- * binary16 when all values lie in [-2048, 2048].  Renders are bit-identical
- * in every encoding. */
+ * binary16 when all values lie in [-2048, 2048], and b - a too when every
+ * x-neighbour difference is exact in binary16 (|b - a| <= 2048, or even).
+ * Renders are bit-identical in every encoding. */
 int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[3], int dtype,
                       int n_frames, rvk_volume** out);
 int rvk_volume_set_frame(rvk_volume* vol, int frame, const void* src, int src_on_device);

@@ -239,9 +239,10 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
   const int ee = kst - (int)floorf(r.exclude_end);
   const float tend = (float)kst, kw0 = (float)(kst - es), kw1 = (float)(kst - ee);
   float t = 0.0f;
-  // skip-rule state as 0.0f / 1.0f: sa = the sample is a back-step, sb =
-  // the last step skipped a sample
-  float sa = 0.0f, sb = 0.0f;
+  // skip-rule state z = sb - sa in {-1, 0, 1}: sa = the sample is a
+  // back-step, sb = the last step skipped a sample (never both), so each arm
+  // updates one float in place
+  float z = 0.0f;
   const bool runs = RUNS && PRECISE && a.ranges != nullptr;
   const RangeWalk w = range_walk(inc);
   // a contributing run must end before the keyhole window
@@ -263,35 +264,35 @@ __device__ __forceinline__ float drr_march(CV vol, const VolGeom& g, const DrrMa
       // reference's single steps; a back-stepped sample adds +0 (acc is
       // never -0), which leaves the sum unchanged without a select
       if (alpha < 0.0f && !in_key) {
-        sa = sb;
-        sb = 0.0f;
+        // sa' = sb, sb' = 0: z' = -sb = median(-z, -1, 0), and sa' = -z'
+        z = __builtin_amdgcn_fmed3f(-z, -1.0f, 0.0f);
         if (!PRECISE) {
-          if (sa != 0.0f) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; }
+          if (z != 0.0f) { p.x = p.x - inc.x; p.y = p.y - inc.y; p.z = p.z - inc.z; }
           else { p.x = p.x + inc.x; p.y = p.y + inc.y; p.z = p.z + inc.z; }
         }
-        t += fmaf(-2.0f, sa, 1.0f);
-        if (VG && sa == 0.0f) visit(q, ts, alpha, gr);
-        // alpha - alpha * sa: alpha, or +0 for a back-stepped sample
-        acc += (Acc)fmaf(-alpha, sa, alpha);
+        t += fmaf(2.0f, z, 1.0f);  // 1 - 2 sa'
+        if (VG && z == 0.0f) visit(q, ts, alpha, gr);
+        // alpha - alpha * sa': alpha, or +0 for a back-stepped sample
+        acc += (Acc)fmaf(alpha, z, alpha);
         if (TERM && acc <= stop_at) break;
         // each further sample of a contributing run steps 1 (sa = sb = 0)
-        if (FV::kFlatRuns && runs && repeat && sa == 0.0f) {
+        if (FV::kFlatRuns && runs && repeat && z == 0.0f) {
           if (flat_run(a, q, p0c, inc, w, cs, ci, alpha, t, t >= kw1 ? tend : flat_lim, acc, stop_at)) break;
         }
       } else {
         if (!PRECISE) {
-          if (sa == 0.0f) step(p, inc);
+          if (z >= 0.0f) step(p, inc);  // sa == 0
           step(p, inc);
         }
-        t += 2.0f - sa;
-        sb = 1.0f - sa;
-        sa = 0.0f;
+        // sa' = 0, sb' = 1 - sa = min(z + 1, 1) (z + 1 >= 0); the step is 2 - sa
+        z = __builtin_amdgcn_fmed3f(z + 1.0f, 0.0f, 1.0f);
+        t += 1.0f + z;
         // each further non-contributing sample steps 2 and leaves sb = 1
         if (runs) {
           const int m = empty_run(a, q, p0c, inc, w, cs, ci, t, tend, 2.0f);
           if (m > 0) {
             t += 2.0f * (float)m;
-            sb = 1.0f;
+            z = 1.0f;
           }
         }
       }

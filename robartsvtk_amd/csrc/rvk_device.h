@@ -110,7 +110,16 @@ template <int LOG2> struct CellOrder {
       // of a volume, (ny + 2)(nz + 2), is below 2^24 like the slab stride)
       return __umul24(__umul24((uint32_t)cz, g.crows) + (uint32_t)cy, g.cys) + (uint32_t)cx;
     } else {
-      return x(cx) + y(cy, g.cys) + z(cz, g.czs);
+      // x(cx) + y(cy) + z(cz) with c & (E - 1) = c - E (c >> L) regrouped:
+      // cx + E cy + E^2 cz + (cx >> L)(E^3 - E) + (cy >> L)(cys - E^2)
+      // + (cz >> L)(czs - E^3): shifts and multiply-adds instead of masks
+      // and ors.  Every term is >= 0 (cys, czs >= E^3) and
+      // the sum is the cell index (< 2^32), so the 24-bit products and the
+      // 32-bit partial sums are exact.
+      constexpr uint32_t E = kEdge;
+      const uint32_t s = ((uint32_t)cz << (2 * LOG2)) + ((uint32_t)cy << LOG2) + (uint32_t)cx;
+      return __umul24((uint32_t)(cz >> LOG2), g.czs - E * E * E) + __umul24((uint32_t)(cy >> LOG2), g.cys - E * E) +
+             __umul24((uint32_t)(cx >> LOG2), E * E * E - E) + s;
     }
   }
 };

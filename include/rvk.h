@@ -322,11 +322,22 @@ int rvk_drr_plan_capture(rvk_drr_plan* plan, float* out, const float* targets, f
  * which is uncentred); a constant image or target gives NCC 0 and no gradient. */
 int rvk_drr_plan_ncc(rvk_drr_plan* plan, const float* images, const float* targets, float* loss,
                      float* dloss_dimage);
+/* Which NCC the plan's loss uses (rvk_drr_plan_ncc, _register and the
+ * registration step): RVK_NCC_CENTERED (default, zero-mean) or
+ * RVK_NCC_UNCENTERED, the reference's own
+ * sum(S T) / (sqrt(sum S^2) sqrt(sum T^2)) with S the rendered image and T
+ * the target (vtkImageNormalizedCrossCorrelation::GetResult,
+ * ImageProcessing/vtkRobartsRegistration/vtkImageNormalizedCrossCorrelation.cxx:203-218).
+ * Drops a captured graph when the mode changes. */
+enum { RVK_NCC_CENTERED = 0, RVK_NCC_UNCENTERED = 1 };
+int rvk_drr_plan_set_ncc(rvk_drr_plan* plan, int mode);
 /* Adjoint of the batched render: given the rendered images and d loss / d
  * image (packed), write d loss / d pose (n_views x 6 floats, pose as in
- * rvk_pose_matrix; needs the plan's dview) and/or d loss / d volume (voxel
- * values, x fastest; float32 volumes with the plain march only).  Rays are
- * re-marched with the forward's control flow; nothing per sample is stored.
+ * rvk_pose_matrix; RVK_ERR_NOT_READY when the plan has no dview) and/or
+ * d loss / d volume (voxel values, x fastest; float32 volumes with the plain
+ * march only; samples in a view's keyhole window are excluded, as in the
+ * forward).  Rays are re-marched with the forward's control flow; nothing per
+ * sample is stored.
  * The volume gradient is a voxel-driven gather (each voxel re-marches only
  * the samples its trilinear support touches) with no atomics; its first call
  * allocates a per-pixel ray record buffer inside the plan.  Gradients are
@@ -337,6 +348,14 @@ int rvk_drr_plan_backward(rvk_drr_plan* plan, const float* images, const float* 
  * targets (loss, n_views floats) and the pose gradient of the loss (n_views x
  * 6 floats), with d loss / d image formed inside the adjoint kernel. */
 int rvk_drr_plan_register(rvk_drr_plan* plan, float* out, const float* targets, float* loss, float* grad_pose);
+
+/* Streaming-read probe (no counterpart in the reference): launches one
+ * coalesced pass that reads every byte of buf (16-byte aligned, a multiple of
+ * 16 bytes) on the context stream; time it with events.  It measures the
+ * device's attainable HBM read bandwidth, the denominator the gather
+ * roofline is checked against.  sink (nullable, device) is written only in
+ * a case no real buffer produces. */
+int rvk_stream_read(rvk_ctx* ctx, const void* buf, size_t bytes, uint32_t* sink);
 
 /* Ray-sample count of a plain march without opacity termination: sum over
  * pixels of floor(num_steps - ray offset) (the DRR's sample count, from the

@@ -61,6 +61,45 @@ void rvk_pose_matrix(const double pose[6], const double center[3], double out[16
 int rvk_posed_view(const rvk_camera* cam, const double pose[6], const double center[3], const double origin[3],
                    const double spacing[3], int width, int height, int flipped, rvk_view* view, float* dm_out);
 
+/* ---- device-resident registration loop --------------------------------------
+ * Replaces the derivative-free vtkPowellMinimizer loop around the registration
+ * metric (ImageProcessing/vtkRobartsRegistration/vtkPowellMinimizer.h:30-117)
+ * with a gradient step that stays on the device: the plan keeps the poses in
+ * device memory and recomputes its views' matrices (and pose derivatives)
+ * from them with the posed-view geometry above, so that one step -- render,
+ * NCC, pose gradient, optimizer update, new views -- is a sequence of kernels
+ * that rvk_drr_plan_capture_step records in one hipGraph.
+ *
+ * RVK_OPT_GD:   pose -= lr * g
+ * RVK_OPT_ADAM: Adam (bias-corrected moments), pose -= lr * m^ / (sqrt(v^) + eps)
+ * lr is lr_rotation for the three angles (rad) and lr_translation for the
+ * three translations (mm).  shared_pose = 1 drives every view with one pose
+ * (the volume's pose seen from several cameras; its gradient is the sum over
+ * the views), 0 gives each view its own pose. */
+enum { RVK_OPT_GD = 0, RVK_OPT_ADAM = 1 };
+typedef struct rvk_pose_optimizer {
+  int32_t kind;
+  int32_t shared_pose;
+  float lr_rotation, lr_translation;
+  float beta1, beta2, eps;
+} rvk_pose_optimizer;
+
+/* Attach device poses to a plan created with pose derivatives: cameras (one
+ * per view; the views keep their sizes and flip), the rotation centre (world
+ * mm), origin (nullable: 0) and poses (6 doubles per pose: 1 pose when
+ * shared_pose, else one per view).  Resets the optimizer state and sets the
+ * views from the poses; drops a captured graph. */
+int rvk_drr_plan_set_poses(rvk_drr_plan* plan, const rvk_camera* cameras, const double center[3],
+                           const double origin[3], const double* poses, const rvk_pose_optimizer* opt);
+/* Current device poses (synchronous). */
+int rvk_drr_plan_get_poses(rvk_drr_plan* plan, double* poses);
+/* One registration step: rvk_drr_plan_register at the current poses (out,
+ * loss and grad_pose are those of the poses before the update), then the
+ * optimizer update and the views at the new poses.  Replays the graph of
+ * rvk_drr_plan_capture_step when called with its buffers. */
+int rvk_drr_plan_step(rvk_drr_plan* plan, float* out, const float* targets, float* loss, float* grad_pose);
+int rvk_drr_plan_capture_step(rvk_drr_plan* plan, float* out, const float* targets, float* loss, float* grad_pose);
+
 /* 4x4 helpers (row major). */
 void rvk_mat4_mul(const double a[16], const double b[16], double out[16]);
 int rvk_mat4_invert(const double a[16], double out[16]);

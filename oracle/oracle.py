@@ -18,7 +18,9 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "librvk_oracle.so"
+# RVK_ORACLE_LIB selects another build of the same source (the CPU baseline
+# times a -march=native build made on the host it runs on)
+LIB_PATH = Path(os.environ.get("RVK_ORACLE_LIB", str(HERE / "librvk_oracle.so")))
 
 U8, I8, U16, I16, U32, I32, F32 = range(7)
 NP2ORC = {np.dtype(np.uint8): U8, np.dtype(np.int8): I8, np.dtype(np.uint16): U16, np.dtype(np.int16): I16,
@@ -100,6 +102,7 @@ def lib() -> C.CDLL:
             ("orc_dual_f32", [P, P, P, P, P, P, P, P, C.c_int], C.c_int),
             ("orc_dual_f64", [P, P, P, P, P, P, P, P, C.c_int], C.c_int),
             ("orc_count_samples", [P, P, P, P, P, C.c_int], C.c_longlong),
+            ("orc_count_drr_evaluated", [P, P, P, P, C.c_int], C.c_longlong),
             ("orc_unsafe_rays", [C.c_int], C.c_longlong),
             ("orc_sample_f64", [P, C.c_double, C.c_double, C.c_double], C.c_double),
             ("orc_sample_f32", [P, C.c_float, C.c_float, C.c_float], C.c_float),
@@ -110,6 +113,14 @@ def lib() -> C.CDLL:
             fn.argtypes = args
             fn.restype = res
     return _lib
+
+
+def use_library(path) -> None:
+    """Load the oracle from another build of the same source (e.g. a
+    -march=native build); the next call binds it."""
+    global _lib, LIB_PATH
+    LIB_PATH = Path(path)
+    _lib = None
 
 
 def default_threads() -> int:
@@ -333,6 +344,16 @@ def form_rays(vol_dims, spacing, view: dict, march: dict, precision: str = "f64"
     assert rc == 0, rc
     return {"start": np.moveaxis(out9[0:3], 0, -1), "inc": np.moveaxis(out9[3:6], 0, -1), "num_steps": out9[6],
             "exclude_start": out9[7], "exclude_end": out9[8], "min_depth": mind, "max_depth": maxd}
+
+
+def count_drr_evaluated(vol: np.ndarray, spacing, view: dict, march: dict, drr_params: dict,
+                        nthreads: int | None = None) -> int:
+    v, _a = _vol(vol, spacing)
+    vw, _z = _view(view)
+    m, d = _march(march), _drr(drr_params)
+    n = lib().orc_count_drr_evaluated(C.byref(v), C.byref(vw), C.byref(m), C.byref(d), nthreads or default_threads())
+    assert n >= 0
+    return int(n)
 
 
 def count_samples(vol: np.ndarray, spacing, view: dict, march: dict, tf1d: dict | None = None,

@@ -1248,6 +1248,28 @@ long long orc_count_samples(const orc_volume* vol, const orc_view* view, const o
   return tot;
 }
 
+long long orc_count_drr_evaluated(const orc_volume* vol, const orc_view* view, const orc_march* mp,
+                                  const orc_drr* dp, int nthreads) {
+  if (!vol || !view || !mp || !dp) return -1;
+  const VolInfo<float> vi = vol_info<float>(vol);
+  const float sl = step_length(vi, mp);
+  std::vector<long long> per_row(view->height, 0);
+  int rc = with_grid(vol, [&](auto g) {
+    parallel_rows(view->height, nthreads, [&](int y) {
+      long long c = 0;
+      Ctl ctl;
+      ctl.count = &c;
+      for (int x = 0; x < view->width; ++x) drr_ray<float>(g, vi, mp, dp, form_ray(vi, view, sl, x, y), x, y, &ctl);
+      per_row[y] = c;
+    });
+    return 0;
+  });
+  if (rc) return -1;
+  long long tot = 0;
+  for (long long c : per_row) tot += c;
+  return tot;
+}
+
 double orc_sample_f64(const orc_volume* vol, double px, double py, double pz) {
   double out = 0.0;
   with_grid(vol, [&](auto g) { out = trilinear<double>(g, px + 0.5, py + 0.5, pz + 0.5); return 0; });

@@ -156,6 +156,11 @@ long long orc_unsafe_rays(int reset);
 long long orc_count_samples(const orc_volume* vol, const orc_view* view, const orc_march* mp,
                             const orc_tf1d* tf1d, const orc_tf2d* tf2d, int nthreads);
 
+/* Trilinear samples the FP32 DRR march takes as configured (with the skip
+ * rule on, the samples it evaluates, fewer than the plain march's). */
+long long orc_count_drr_evaluated(const orc_volume* vol, const orc_view* view, const orc_march* mp,
+                                  const orc_drr* dp, int nthreads);
+
 /* Trilinear sample at position p (voxel index space, half-texel convention). */
 double orc_sample_f64(const orc_volume* vol, double px, double py, double pz);
 float orc_sample_f32(const orc_volume* vol, float px, float py, float pz);

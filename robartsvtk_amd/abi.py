@@ -78,6 +78,15 @@ class Camera(C.Structure):
                 ("parallel_scale", C.c_double), ("use_horizontal_view_angle", C.c_int32)]
 
 
+class PoseOptimizer(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("shared_pose", C.c_int32), ("lr_rotation", C.c_float),
+                ("lr_translation", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
+
+
+OPT_GD, OPT_ADAM = 0, 1
+NCC_CENTERED, NCC_UNCENTERED = 0, 1
+
+
 _P = C.c_void_p
 _I = C.c_int
 _SIGS = {
@@ -118,6 +127,7 @@ _SIGS = {
     "rvk_post_shade": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
     "rvk_post_shade_inex": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
     "rvk_count_ray_steps": (_I, [_P, _P, C.POINTER(View), C.POINTER(MarchParams), C.POINTER(C.c_int64)]),
+    "rvk_stream_read": (_I, [_P, _P, C.c_size_t, _P]),
     "rvk_drr_plan_create": (_I, [_P, _P, _I, _I, C.POINTER(View), _P, C.POINTER(MarchParams), C.POINTER(DrrParams),
                                  C.POINTER(_P)]),
     "rvk_drr_plan_destroy": (_I, [_P]),
@@ -131,6 +141,12 @@ _SIGS = {
     "rvk_drr_plan_ncc": (_I, [_P, _P, _P, _P, _P]),
     "rvk_drr_plan_backward": (_I, [_P, _P, _P, _P, _P]),
     "rvk_drr_plan_register": (_I, [_P, _P, _P, _P, _P]),
+    "rvk_drr_plan_set_ncc": (_I, [_P, _I]),
+    "rvk_drr_plan_set_poses": (_I, [_P, _P, C.POINTER(C.c_double), _P, C.POINTER(C.c_double),
+                                    C.POINTER(PoseOptimizer)]),
+    "rvk_drr_plan_get_poses": (_I, [_P, C.POINTER(C.c_double)]),
+    "rvk_drr_plan_step": (_I, [_P, _P, _P, _P, _P]),
+    "rvk_drr_plan_capture_step": (_I, [_P, _P, _P, _P, _P]),
     "rvk_compute_view": (_I, [C.POINTER(Camera), _P, C.POINTER(C.c_double), C.POINTER(C.c_double), _I, _I, _I,
                               C.POINTER(View)]),
     "rvk_compute_matrices": (_I, [C.POINTER(Camera), _P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
@@ -393,6 +409,12 @@ def render_dual(ctx: Context, vol1: Volume, vol2: Volume, tf: TF2D, params: Dual
                                 C.byref(aux) if aux is not None else None))
 
 
+def stream_read(ctx: Context, buf, sink=None) -> None:
+    """rvk_stream_read over a torch tensor (device), on the context stream."""
+    check(lib().rvk_stream_read(ctx.handle, C.c_void_p(dptr(buf)), buf.numel() * buf.element_size(),
+                                None if sink is None else C.c_void_p(dptr(sink))))
+
+
 def count_ray_steps(ctx: Context, vol: Volume, view: View, march: MarchParams) -> int:
     n = C.c_int64(0)
     check(lib().rvk_count_ray_steps(ctx.handle, vol.handle, C.byref(view), C.byref(march), C.byref(n)))
@@ -461,6 +483,31 @@ class DrrPlan:
     def register(self, out, targets, loss, grad_pose):
         p = lambda t: C.c_void_p(dptr(t))
         check(lib().rvk_drr_plan_register(self.handle, p(out), p(targets), p(loss), p(grad_pose)))
+
+    def set_ncc(self, mode: int):
+        check(lib().rvk_drr_plan_set_ncc(self.handle, mode))
+
+    def set_poses(self, cameras, center, poses, optimizer: PoseOptimizer, origin=None):
+        """Device-resident poses (one per view, or one shared): cameras per
+        view, rotation centre (mm), poses n x 6 (rx, ry, rz rad, t mm)."""
+        cams = (Camera * len(cameras))(*cameras)
+        ps = np.ascontiguousarray(poses, np.float64).ravel()
+        check(lib().rvk_drr_plan_set_poses(self.handle, cams, dvec(center), None if origin is None else dvec(origin),
+                                           ps.ctypes.data_as(C.POINTER(C.c_double)), C.byref(optimizer)))
+        self.n_pose_slots = 1 if optimizer.shared_pose else self.n_views
+
+    def poses(self) -> np.ndarray:
+        out = np.zeros(6 * self.n_pose_slots, np.float64)
+        check(lib().rvk_drr_plan_get_poses(self.handle, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(-1, 6)
+
+    def step(self, out, targets, loss, grad_pose):
+        p = lambda t: C.c_void_p(dptr(t))
+        check(lib().rvk_drr_plan_step(self.handle, p(out), p(targets), p(loss), p(grad_pose)))
+
+    def capture_step(self, out, targets, loss, grad_pose):
+        p = lambda t: C.c_void_p(dptr(t))
+        check(lib().rvk_drr_plan_capture_step(self.handle, p(out), p(targets), p(loss), p(grad_pose)))
 
     def split(self, packed):
         """Per-view [H, W] views of a packed torch tensor."""

@@ -13,8 +13,12 @@ the differentiable inputs.
 
 The forward is rvk_drr_plan_render and the backward rvk_drr_plan_backward
 (pose adjoint and, when the volume requires grad, the voxel-driven volume
-adjoint).  Nothing here computes on the CPU except the per-view camera
-matrices and their pose derivatives (rvk_posed_view, double precision).
+adjoint).  The plan holds one set of views and one volume frame; every forward
+records which poses and volume it rendered, and its backward loads them back
+into the plan when another forward has replaced them since (several renders in
+one loss, line searches).  Nothing here computes on the CPU except the
+per-view camera matrices and their pose derivatives (rvk_posed_view, double
+precision).
 """
 from __future__ import annotations
 
@@ -35,17 +39,22 @@ class DrrRenderer:
         self.sizes = [(int(w), int(h)) for w, h in sizes]
         self.center = np.asarray(center, np.float64)
         if isinstance(volume, np.ndarray):
-            self.vol = abi.Volume.from_array(self.ctx, volume.astype(np.float32, copy=False), self.spacing)
-            self.dims_zyx = volume.shape
+            base = torch.from_numpy(np.ascontiguousarray(volume, np.float32)).cuda()
         else:
-            t = volume.detach().contiguous()
-            if t.dtype != torch.float32 or not t.is_cuda:
+            base = volume.detach()
+            if base.dtype != torch.float32 or not base.is_cuda:
                 raise ValueError("volume tensor must be float32 on the HIP device")
-            self.dims_zyx = tuple(t.shape)
-            self.vol = abi.Volume(self.ctx, (t.shape[2], t.shape[1], t.shape[0]), self.spacing, np.float32)
-            self.vol.set_frame(0, t)
-        views, dview = self.views_for(np.zeros((len(self.cameras), 6)))
+            base = base.contiguous().clone()
+        # the volume renders fall back to when a forward passes none
+        self.base_volume = base
+        self.dims_zyx = tuple(base.shape)
+        self.vol = abi.Volume(self.ctx, (base.shape[2], base.shape[1], base.shape[0]), self.spacing, np.float32)
+        self.vol.set_frame(0, base)
+        self._loaded_volume = base
+        self._loaded_poses = np.zeros((len(self.cameras), 6))
+        views, dview = self.views_for(self._loaded_poses)
         self.plan = abi.DrrPlan(self.ctx, self.vol, views, march, drr, dview=dview)
+        self.generation = 0  # bumped whenever the plan's views or frame change
 
     @property
     def n_views(self) -> int:
@@ -59,6 +68,21 @@ class DrrRenderer:
             dms.append(dm)
         return views, np.stack(dms)
 
+    def load(self, poses: np.ndarray, volume: torch.Tensor) -> int:
+        """Put `poses` (n_views x 6, float64) and `volume` into the plan; returns
+        the generation that identifies this state."""
+        poses = np.array(poses, np.float64).reshape(self.n_views, 6)
+        if not np.array_equal(poses, self._loaded_poses):
+            views, dview = self.views_for(poses)
+            self.plan.update_views(views, dview)
+            self._loaded_poses = poses
+            self.generation += 1
+        if volume is not self._loaded_volume:
+            self.vol.set_frame(0, volume)
+            self._loaded_volume = volume
+            self.generation += 1
+        return self.generation
+
     def close(self) -> None:
         self.plan.close()
         self.vol.close()
@@ -67,15 +91,14 @@ class DrrRenderer:
 class DrrFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, poses: torch.Tensor, volume: torch.Tensor | None, renderer: DrrRenderer):
-        p = poses.detach().to("cpu", torch.float64).numpy().reshape(renderer.n_views, 6)
-        views, dview = renderer.views_for(p)
-        renderer.plan.update_views(views, dview)
-        if volume is not None:
-            renderer.vol.set_frame(0, volume.detach().contiguous())
+        p = poses.detach().to("cpu", torch.float64).numpy().reshape(renderer.n_views, 6).copy()
+        vol = renderer.base_volume if volume is None else volume.detach().contiguous()
+        gen = renderer.load(p, vol)
         out = torch.empty(renderer.plan.total, dtype=torch.float32, device="cuda")
         renderer.plan.render(out)
         ctx.renderer = renderer
         ctx.poses_meta = (poses.dtype, poses.device, poses.shape)
+        ctx.state = (gen, p, vol)
         ctx.has_volume = volume is not None
         ctx.save_for_backward(out)
         return out
@@ -84,6 +107,10 @@ class DrrFunction(torch.autograd.Function):
     def backward(ctx, grad_out: torch.Tensor):
         (out,) = ctx.saved_tensors
         r: DrrRenderer = ctx.renderer
+        gen, p, vol = ctx.state
+        if r.generation != gen:
+            # another forward has rendered other poses or another volume since
+            r.load(p, vol)
         want_pose = ctx.needs_input_grad[0]
         want_vol = ctx.has_volume and ctx.needs_input_grad[1]
         gp = torch.empty(6 * r.n_views, dtype=torch.float32, device="cuda") if want_pose else None
@@ -99,5 +126,5 @@ class DrrFunction(torch.autograd.Function):
 def drr_render(renderer: DrrRenderer, poses: torch.Tensor, volume: torch.Tensor | None = None) -> torch.Tensor:
     """Packed DRR images of all views at `poses` (n_views x 6); differentiable
     in `poses` and, when given, in `volume` (float32 [z, y, x] CUDA tensor
-    that replaces the renderer's volume for this call)."""
+    rendered in place of the renderer's own volume for this call)."""
     return DrrFunction.apply(poses, volume, renderer)

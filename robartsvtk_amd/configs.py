@@ -245,7 +245,9 @@ def scene_c4(scale: float = 1.0, seed: int = 4, n_poses: int = 64, with_derivati
             views.append(abi.posed_view(cam, poses[i], c, s, s, spacing=spacing))
         sizes.append(s)
         cams.append(cam)
-    march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": 1, "precise": 1}
+    # the batched, differentiable DRR marches without the skip rule: which
+    # samples it skips jumps with the pose (the single-view c2 keeps it)
+    march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": 0, "precise": 1}
     drr = {"intercept": 0.02, "slope": 2e-5, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L, "ref_termination": 0,
            "tint": (0, 0, 0, 0)}
     return Scene("c4", vol, spacing, views, march, drr=drr,

@@ -18,6 +18,7 @@
 
 #include "rvk_internal.h"
 #include "rvk_march.h"
+#include "rvk_pose_math.h"
 
 namespace rvk {
 
@@ -156,11 +157,15 @@ __global__ void reset_queues_kernel(unsigned* q) {
 }
 
 // ---------------------------------------------------------------------------
-// Zero-mean NCC.  Per tile: raw moments in double; per view: combine.
+// NCC per view, zero-mean (Pearson) or the reference's uncentred
+// sum(I T) / (sqrt(sum I^2) sqrt(sum T^2))
+// (ImageProcessing/vtkRobartsRegistration/vtkImageNormalizedCrossCorrelation.cxx:116-128,203-218).
+// Per tile: raw moments in double; per view: combine.
 struct NccArgs {
   const BatchView* views;
   const int* item_begin;
   int n_views, n_items;
+  int centred;
   const float* img;
   const float* tgt;
   double* part;    // n_items x 5
@@ -218,10 +223,11 @@ __global__ __launch_bounds__(256) void ncc_view_kernel(NccArgs a) {
   }
   if (threadIdx.x == 0) {
     const double n = (double)a.views[v].v.W * (double)a.views[v].v.H;
-    const double mi = red[0][0] / n, mt = red[1][0] / n;
+    // the uncentred form is the centred one with both means taken as 0
+    const double mi = a.centred ? red[0][0] / n : 0.0, mt = a.centred ? red[1][0] / n : 0.0;
     const double saa = red[2][0] - n * mi * mi, sbb = red[3][0] - n * mt * mt, sab = red[4][0] - n * mi * mt;
     const bool ok = saa > 0.0 && sbb > 0.0;
-    const double inv = ok ? 1.0 / sqrt(saa * sbb) : 0.0;
+    const double inv = !ok ? 0.0 : a.centred ? 1.0 / sqrt(saa * sbb) : 1.0 / (sqrt(saa) * sqrt(sbb));
     const double ncc = ok ? sab * inv : 0.0;
     double* st = a.stats + 8 * v;
     st[0] = mi; st[1] = mt; st[2] = inv; st[3] = ok ? ncc / saa : 0.0; st[4] = ncc; st[5] = n; st[6] = ok;
@@ -379,6 +385,7 @@ struct RayRec {
   int n;
   float cs, ci;   // alpha = cs * value + ci
   float coef;     // d loss / d L_p * ds
+  float kw0, kw1; // keyhole window: samples t in [kw0, kw1) are never accumulated
 };
 
 struct VolGradArgs {
@@ -414,6 +421,10 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
   rr.n = step_count(r.num_steps - j);
   // the rays the precise forward march takes no samples of (drr_march)
   if (!samples_in_cells(g, V3<float>{rr.p0x, rr.p0y, rr.p0z}, r.inc, rr.n)) rr.n = 0;
+  // the keyhole window exactly as drr_march forms it (t = kst - k)
+  const int es = rr.n - (int)ceilf(r.exclude_start), ee = rr.n - (int)floorf(r.exclude_end);
+  rr.kw0 = (float)(rr.n - es);
+  rr.kw1 = (float)(rr.n - ee);
   rr.ci = a.b.m.intercept * -ds;
   rr.cs = a.b.m.slope * -ds;
   // d loss / d mu at a sample = (d loss / d L) * ds * slope, with dL/dI = -I for exp(-L)
@@ -423,6 +434,16 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
 }
 
 __device__ __forceinline__ float hat(float t) { return fmaxf(0.0f, 1.0f - fabsf(t)); }
+
+// Trilinear weight, along one axis, of voxel i (of n) at the cell coordinate
+// c: the hat centred on c = i + 1, except that the edge cells hold a clamped
+// voxel twice (cell 0 = voxels (0, 0), cell n = (n - 1, n - 1)), so voxel 0
+// weighs 1 for c < 1 and voxel n - 1 weighs 1 for c > n.
+__device__ __forceinline__ float axis_weight(float c, int i, int n) {
+  const float ci = (float)(i + 1);
+  if ((i == 0 && c < ci) || (i == n - 1 && c > ci)) return 1.0f;
+  return hat(c - ci);
+}
 
 template <int L>
 __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
@@ -451,20 +472,26 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
   }
   float acc = 0.0f;
   if (live) {
-    // the support in sample coordinates p: (i-0.5, i+1.5)^3; lo is its low
-    // corner, cl the same corner in cell coordinates c = p + 0.5, where the
-    // voxel's centre is at i + 1
-    const float lo[3] = {(float)i - 0.5f, (float)jj - 0.5f, (float)k - 0.5f};
-    const float cl[3] = {(float)i, (float)jj, (float)k};
+    // the support in cell coordinates c = p + 0.5: (i, i+2) per axis around
+    // the voxel's centre i + 1, reaching down to the cells below c = 1 for
+    // voxel 0 and up to those above c = n for voxel n - 1 (edge cells hold
+    // the clamped voxel twice)
+    const int vi[3] = {i, jj, k};
+    float clo[3], chi[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      clo[ax] = vi[ax] == 0 ? -1.0f : (float)vi[ax];
+      chi[ax] = vi[ax] == g.n[ax] - 1 ? (float)(g.n[ax] + 2) : (float)(vi[ax] + 2);
+    }
     for (int v = 0; v < a.b.n_views; ++v) {
       const BatchView& bv = a.b.views[v];
       const int W = bv.v.W, H = bv.v.H;
-      // pixel box of the support's 8 corners
+      // pixel box of the support's 8 corners (in sample coordinates p = c - 0.5)
       float xmin = 1e30f, xmax = -1e30f, ymin = 1e30f, ymax = -1e30f;
       bool behind = false;
       for (int c = 0; c < 8; ++c) {
-        const float px = lo[0] + ((c & 1) ? 2.0f : 0.0f), py = lo[1] + ((c & 2) ? 2.0f : 0.0f);
-        const float pz = lo[2] + ((c & 4) ? 2.0f : 0.0f);
+        const float px = ((c & 1) ? chi[0] : clo[0]) - 0.5f, py = ((c & 2) ? chi[1] : clo[1]) - 0.5f;
+        const float pz = ((c & 4) ? chi[2] : clo[2]) - 0.5f;
         const float* M = bv.inv;
         const float hx = M[0] * px + M[1] * py + M[2] * pz + M[3];
         const float hy = M[4] * px + M[5] * py + M[6] * pz + M[7];
@@ -491,18 +518,20 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
 #pragma unroll
           for (int ax = 0; ax < 3; ++ax) {
             if (in[ax] != 0.0f) {
-              const float ta = (cl[ax] - p0[ax]) / in[ax], tb = (cl[ax] + 2.0f - p0[ax]) / in[ax];
+              const float ta = (clo[ax] - p0[ax]) / in[ax], tb = (chi[ax] - p0[ax]) / in[ax];
               tlo = fmaxf(tlo, fminf(ta, tb));
               thi = fminf(thi, fmaxf(ta, tb));
-            } else if (p0[ax] <= cl[ax] || p0[ax] >= cl[ax] + 2.0f) {
+            } else if (p0[ax] <= clo[ax] || p0[ax] >= chi[ax]) {
               thi = -1.0f;
             }
           }
           const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(rr.n - 1, (int)ceilf(thi) + 1);
           for (int m = m0; m <= m1; ++m) {
             const float t = (float)m;
+            // samples in the keyhole window are not accumulated by the forward
+            if (t >= rr.kw0 && t < rr.kw1) continue;
             const float qx = fmaf(t, rr.ix, rr.p0x), qy = fmaf(t, rr.iy, rr.p0y), qz = fmaf(t, rr.iz, rr.p0z);
-            const float w = hat(qx - (cl[0] + 1.0f)) * hat(qy - (cl[1] + 1.0f)) * hat(qz - (cl[2] + 1.0f));
+            const float w = axis_weight(qx, i, g.n[0]) * axis_weight(qy, jj, g.n[1]) * axis_weight(qz, k, g.n[2]);
             if (w <= 0.0f) continue;
             const float alpha = fmaf(rr.cs, trilinear(vol, g, qx, qy, qz), rr.ci);
             if (alpha < 0.0f) acc += rr.coef * w;
@@ -511,6 +540,67 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
     }
   }
   a.grad[vid] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident poses: the registration loop's pose update and the views'
+// matrices recomputed from the poses, so that one replayed graph holds the
+// whole iteration (render, NCC, pose gradient, update).
+struct PoseGeom {
+  double pv[16];      // the view's camera transform P V (its aspect)
+  double vox[16];     // voxel -> world
+  double center[3];   // rotation centre (world mm)
+  int slot;           // which pose drives this view
+  int pad;
+};
+
+__global__ __launch_bounds__(64) void pose_views_kernel(const PoseGeom* __restrict__ geo, const double* poses,
+                                                        BatchView* views, int n_views) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_views) return;
+  const PoseGeom& G = geo[v];
+  float m[16], dm[96];
+  // a singular posed matrix (no rigid pose gives one) leaves the view as it was
+  if (!pose::posed_view(G.pv, G.vox, poses + 6 * G.slot, G.center, m, dm)) return;
+  BatchView& b = views[v];
+  for (int k = 0; k < 16; ++k) b.v.m[k] = m[k];
+  for (int k = 0; k < 96; ++k) b.dm[k] = dm[k];
+  pose::baked_inverse(m, b.inv);
+}
+
+struct PoseUpdateArgs {
+  rvk_pose_optimizer o;
+  int n_views, n_slots;
+};
+
+// One workgroup.  The gradient of slot s's pose is the sum of d loss / d pose
+// over the views it drives (one view each with per-view poses).  state holds
+// the Adam moments m[n_slots*6], v[n_slots*6] and the step count.
+__global__ __launch_bounds__(256) void pose_update_kernel(PoseUpdateArgs a, const PoseGeom* __restrict__ geo,
+                                                          const float* __restrict__ grad, double* poses,
+                                                          double* state) {
+  const int nd = a.n_slots * 6;
+  const double t = state[2 * nd] + 1.0;
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+    const int slot = i / 6, d = i % 6;
+    double g = 0.0;
+    for (int v = 0; v < a.n_views; ++v)
+      if (geo[v].slot == slot) g += (double)grad[6 * v + d];
+    const double lr = d < 3 ? a.o.lr_rotation : a.o.lr_translation;
+    if (a.o.kind == RVK_OPT_ADAM) {
+      const double b1 = a.o.beta1, b2 = a.o.beta2;
+      const double m = b1 * state[i] + (1.0 - b1) * g;
+      const double s = b2 * state[nd + i] + (1.0 - b2) * g * g;
+      state[i] = m;
+      state[nd + i] = s;
+      const double mh = m / (1.0 - pow(b1, t)), sh = s / (1.0 - pow(b2, t));
+      poses[i] -= lr * mh / (sqrt(sh) + (double)a.o.eps);
+    } else {
+      poses[i] -= lr * g;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) state[2 * nd] = t;
 }
 
 }  // namespace rvk
@@ -551,6 +641,14 @@ struct rvk_drr_plan {
   float* g_loss = nullptr;
   float* g_grad = nullptr;
   int g_enc = 0;  // cell encoding of the frame when the graph was captured
+  bool g_step = false;  // the graph is a registration step with the pose update
+  int ncc_mode = RVK_NCC_CENTERED;
+  // device-resident poses (rvk_drr_plan_set_poses)
+  PoseGeom* d_pose_geom = nullptr;  // per view
+  double* d_poses = nullptr;        // n_slots x 6
+  double* d_opt_state = nullptr;    // 2 x n_slots x 6 moments + the step count
+  int n_slots = 0;
+  rvk_pose_optimizer opt{};
 };
 
 namespace {
@@ -569,30 +667,8 @@ int fill_views(rvk_drr_plan* p, const rvk_view* views, const float* dview, std::
     std::memset(&b, 0, sizeof(b));
     b.v = view_geom(&views[v]);
     if (dview) std::memcpy(b.dm, dview + 96 * (size_t)v, sizeof(b.dm));
-    double m[16], mi[16];
-    for (int k = 0; k < 16; ++k) m[k] = views[v].view_to_voxels[k];
     // inverse of the baked matrix (Gauss-Jordan, partial pivoting)
-    double aug[4][8];
-    for (int r = 0; r < 4; ++r)
-      for (int c = 0; c < 4; ++c) { aug[r][c] = m[4 * r + c]; aug[r][4 + c] = (r == c); }
-    bool singular = false;
-    for (int c = 0; c < 4 && !singular; ++c) {
-      int pv = c;
-      for (int r = c + 1; r < 4; ++r)
-        if (std::fabs(aug[r][c]) > std::fabs(aug[pv][c])) pv = r;
-      if (aug[pv][c] == 0.0) { singular = true; break; }
-      for (int k = 0; k < 8; ++k) std::swap(aug[c][k], aug[pv][k]);
-      const double iv = 1.0 / aug[c][c];
-      for (int k = 0; k < 8; ++k) aug[c][k] *= iv;
-      for (int r = 0; r < 4; ++r)
-        if (r != c && aug[r][c] != 0.0) {
-          const double f = aug[r][c];
-          for (int k = 0; k < 8; ++k) aug[r][k] -= f * aug[c][k];
-        }
-    }
-    for (int r = 0; r < 4; ++r)
-      for (int c = 0; c < 4; ++c) mi[4 * r + c] = singular ? 0.0 : aug[r][4 + c];
-    for (int k = 0; k < 16; ++k) b.inv[k] = (float)mi[k];
+    pose::baked_inverse(views[v].view_to_voxels, b.inv);
     b.tiles16_x = (views[v].width + 15) / 16;
   }
   return RVK_OK;
@@ -675,6 +751,7 @@ NccArgs ncc_args(const rvk_drr_plan* p, const float* img, const float* tgt, floa
   n.stats = p->d_stats;
   n.loss = loss;
   n.grad_img = grad_img;
+  n.centred = p->ncc_mode == RVK_NCC_CENTERED;
   return n;
 }
 
@@ -716,6 +793,27 @@ void drop_graph(rvk_drr_plan* p) {
   p->g_tgt = nullptr;
   p->g_loss = nullptr;
   p->g_grad = nullptr;
+  p->g_step = false;
+}
+
+// The pose update and the views' matrices from the new poses.
+int launch_pose_update(rvk_drr_plan* p, const float* grad, hipStream_t s) {
+  PoseUpdateArgs a;
+  a.o = p->opt;
+  a.n_views = p->n_views;
+  a.n_slots = p->n_slots;
+  hipLaunchKernelGGL(pose_update_kernel, dim3(1), dim3(256), 0, s, a, p->d_pose_geom, grad, p->d_poses,
+                     p->d_opt_state);
+  hipLaunchKernelGGL(pose_views_kernel, dim3((p->n_views + 63) / 64), dim3(64), 0, s, p->d_pose_geom, p->d_poses,
+                     p->d_views, p->n_views);
+  RVK_HIP(hipGetLastError());
+  return RVK_OK;
+}
+
+int launch_step(rvk_drr_plan* p, float* out, const float* tgt, float* loss, float* grad, hipStream_t s) {
+  int rc = launch_register(p, out, tgt, loss, grad, s);
+  if (rc) return rc;
+  return launch_pose_update(p, grad, s);
 }
 
 }  // namespace
@@ -798,6 +896,9 @@ int rvk_drr_plan_destroy(rvk_drr_plan* p) {
   hipFree(p->d_pose_part);
   if (p->d_rays) hipFree(p->d_rays);
   if (p->d_jac) hipFree(p->d_jac);
+  if (p->d_pose_geom) hipFree(p->d_pose_geom);
+  if (p->d_poses) hipFree(p->d_poses);
+  if (p->d_opt_state) hipFree(p->d_opt_state);
   delete p;
   return RVK_OK;
 }
@@ -886,7 +987,7 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
 int rvk_drr_plan_render(rvk_drr_plan* p, float* out) {
   if (!p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && p->g_out == out && !p->g_tgt && p->g_enc == p->vol->enc[p->frame]) {
+  if (p->exec && !p->g_step && p->g_out == out && !p->g_tgt && p->g_enc == p->vol->enc[p->frame]) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }
@@ -924,7 +1025,7 @@ int rvk_drr_plan_register(rvk_drr_plan* p, float* out, const float* tgt, float* 
   if (!p || !out || !tgt || !loss || !grad) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad &&
+  if (p->exec && !p->g_step && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad &&
       p->g_enc == p->vol->enc[p->frame]) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
@@ -948,6 +1049,8 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
                           float* grad_vol) {
   if (!p || !images || !dldi) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
+  // the pose tangents come from the plan's dview (d_jac exists exactly then)
+  if (grad_pose && !p->d_jac) return fail(RVK_ERR_NOT_READY, "the plan was created without pose derivatives (dview)");
   RVK_HIP(hipSetDevice(p->ctx->device));
   hipStream_t s = p->ctx->stream;
   const int blocks = (p->n_items + 3) / 4;
@@ -992,6 +1095,116 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
                          va);
     RVK_HIP(hipGetLastError());
   }
+  return RVK_OK;
+}
+
+int rvk_drr_plan_set_ncc(rvk_drr_plan* p, int mode) {
+  if (!p) return fail(RVK_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (mode != RVK_NCC_CENTERED && mode != RVK_NCC_UNCENTERED)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown NCC mode %d", mode);
+  if (mode != p->ncc_mode) drop_graph(p);
+  p->ncc_mode = mode;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_set_poses(rvk_drr_plan* p, const rvk_camera* cameras, const double center[3], const double origin[3],
+                           const double* poses, const rvk_pose_optimizer* opt) {
+  if (!p || !cameras || !center || !poses || !opt) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->d_jac) return fail(RVK_ERR_NOT_READY, "the plan was created without pose derivatives (dview)");
+  if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
+  if (opt->kind != RVK_OPT_GD && opt->kind != RVK_OPT_ADAM)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown optimizer kind %d", opt->kind);
+  if (!std::isfinite(opt->lr_rotation) || !std::isfinite(opt->lr_translation) || opt->lr_rotation < 0.0f ||
+      opt->lr_translation < 0.0f)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "learning rates must be finite and >= 0");
+  if (opt->kind == RVK_OPT_ADAM && !(opt->beta1 >= 0.0f && opt->beta1 < 1.0f && opt->beta2 >= 0.0f &&
+                                     opt->beta2 < 1.0f && opt->eps > 0.0f))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "Adam needs beta1, beta2 in [0, 1) and eps > 0");
+  const int n_slots = opt->shared_pose ? 1 : p->n_views;
+  const double zero[3] = {0.0, 0.0, 0.0};
+  std::vector<PoseGeom> geo(p->n_views);
+  std::vector<BatchView> hv(p->n_views);
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  RVK_HIP(hipStreamSynchronize(p->ctx->stream));
+  RVK_HIP(hipMemcpy(hv.data(), p->d_views, sizeof(BatchView) * p->n_views, hipMemcpyDeviceToHost));
+  for (int v = 0; v < p->n_views; ++v) {
+    PoseGeom& g = geo[v];
+    std::memset(&g, 0, sizeof(g));
+    const double aspect = (double)hv[v].v.W / (double)hv[v].v.H;
+    if (camera_pv(&cameras[v], aspect, g.pv)) return fail(RVK_ERR_INVALID_ARGUMENT, "camera %d is invalid", v);
+    voxels_matrix(origin ? origin : zero, p->vol->spacing, g.vox);
+    for (int a = 0; a < 3; ++a) g.center[a] = center[a];
+    g.slot = opt->shared_pose ? 0 : v;
+    float m[16];
+    if (!pose::posed_view(g.pv, g.vox, poses + 6 * g.slot, g.center, m, nullptr))
+      return fail(RVK_ERR_INVALID_ARGUMENT, "view %d: singular posed matrix", v);
+  }
+  if (n_slots != p->n_slots) {
+    if (p->d_poses) hipFree(p->d_poses);
+    if (p->d_opt_state) hipFree(p->d_opt_state);
+    p->d_poses = nullptr;
+    p->d_opt_state = nullptr;
+    p->n_slots = 0;
+    RVK_HIP(hipMalloc((void**)&p->d_poses, sizeof(double) * 6 * n_slots));
+    RVK_HIP(hipMalloc((void**)&p->d_opt_state, sizeof(double) * (12 * (size_t)n_slots + 1)));
+    p->n_slots = n_slots;
+  }
+  if (!p->d_pose_geom) RVK_HIP(hipMalloc((void**)&p->d_pose_geom, sizeof(PoseGeom) * p->n_views));
+  drop_graph(p);
+  p->opt = *opt;
+  const std::vector<double> state(12 * (size_t)n_slots + 1, 0.0);
+  RVK_HIP(hipMemcpy(p->d_pose_geom, geo.data(), sizeof(PoseGeom) * p->n_views, hipMemcpyHostToDevice));
+  RVK_HIP(hipMemcpy(p->d_poses, poses, sizeof(double) * 6 * n_slots, hipMemcpyHostToDevice));
+  RVK_HIP(hipMemcpy(p->d_opt_state, state.data(), sizeof(double) * state.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(pose_views_kernel, dim3((p->n_views + 63) / 64), dim3(64), 0, p->ctx->stream, p->d_pose_geom,
+                     p->d_poses, p->d_views, p->n_views);
+  RVK_HIP(hipGetLastError());
+  RVK_HIP(hipStreamSynchronize(p->ctx->stream));
+  return RVK_OK;
+}
+
+int rvk_drr_plan_get_poses(rvk_drr_plan* p, double* poses) {
+  if (!p || !poses) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->d_poses) return fail(RVK_ERR_NOT_READY, "the plan has no device poses (rvk_drr_plan_set_poses)");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  RVK_HIP(hipMemcpyAsync(poses, p->d_poses, sizeof(double) * 6 * p->n_slots, hipMemcpyDeviceToHost, p->ctx->stream));
+  RVK_HIP(hipStreamSynchronize(p->ctx->stream));
+  return RVK_OK;
+}
+
+int rvk_drr_plan_step(rvk_drr_plan* p, float* out, const float* tgt, float* loss, float* grad) {
+  if (!p || !out || !tgt || !loss || !grad) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->d_poses) return fail(RVK_ERR_NOT_READY, "the plan has no device poses (rvk_drr_plan_set_poses)");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  if (p->exec && p->g_step && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad &&
+      p->g_enc == p->vol->enc[p->frame]) {
+    RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
+    return RVK_OK;
+  }
+  return launch_step(p, out, tgt, loss, grad, p->ctx->stream);
+}
+
+int rvk_drr_plan_capture_step(rvk_drr_plan* p, float* out, const float* tgt, float* loss, float* grad) {
+  if (!p || !out || !tgt || !loss || !grad) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->d_poses) return fail(RVK_ERR_NOT_READY, "the plan has no device poses (rvk_drr_plan_set_poses)");
+  RVK_HIP(hipSetDevice(p->ctx->device));
+  drop_graph(p);
+  RVK_HIP(hipStreamSynchronize(p->ctx->stream));
+  RVK_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_step(p, out, tgt, loss, grad, p->cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  p->graph = g;
+  e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) { drop_graph(p); return hip_fail(e, "hipGraphInstantiate"); }
+  p->g_out = out;
+  p->g_tgt = tgt;
+  p->g_loss = loss;
+  p->g_grad = grad;
+  p->g_enc = p->vol->enc[p->frame];
+  p->g_step = true;
   return RVK_OK;
 }
 

@@ -97,16 +97,28 @@ void voxels_to_world(const double* user, const double origin[3], const double sp
   else std::memcpy(m, vox, sizeof(vox));
 }
 
-void rot(int axis, double ang, double m[16], double dm[16]) {
-  identity(m);
-  std::memset(dm, 0, 16 * sizeof(double));
-  const double c = std::cos(ang), s = std::sin(ang);
-  const int a = (axis + 1) % 3, b = (axis + 2) % 3;
-  m[5 * a] = c; m[4 * a + b] = -s; m[4 * b + a] = s; m[5 * b] = c;
-  dm[5 * a] = -s; dm[4 * a + b] = -c; dm[4 * b + a] = c; dm[5 * b] = -s;
+}  // namespace
+
+namespace rvk {
+
+// P V of a camera (the posed-view geometry composes it with the pose and the
+// voxel matrix, see rvk_pose_math.h).
+int camera_pv(const rvk_camera* cam, double aspect, double pv[16]) {
+  if (!cam || !(aspect > 0.0)) return RVK_ERR_INVALID_ARGUMENT;
+  if (!cam->parallel_projection && !(cam->clipping_range[0] > 0.0)) return RVK_ERR_INVALID_ARGUMENT;
+  if (!(cam->clipping_range[1] > cam->clipping_range[0])) return RVK_ERR_INVALID_ARGUMENT;
+  double p[16], v[16];
+  projection_matrix(cam, aspect, p);
+  view_matrix(cam, v);
+  rvk_mat4_mul(p, v, pv);
+  return RVK_OK;
 }
 
-}  // namespace
+void voxels_matrix(const double origin[3], const double spacing[3], double vox[16]) {
+  voxels_to_world(nullptr, origin, spacing, vox);
+}
+
+}  // namespace rvk
 
 extern "C" {
 
@@ -186,80 +198,6 @@ int rvk_compute_view(const rvk_camera* cam, const double user[16], const double 
   view->width = width;
   view->height = height;
   view->flipped = flipped;
-  return RVK_OK;
-}
-
-void rvk_pose_matrix(const double pose[6], const double c[3], double out[16], double* dout) {
-  double rx[16], ry[16], rz[16], drx[16], dry[16], drz[16];
-  rot(0, pose[0], rx, drx);
-  rot(1, pose[1], ry, dry);
-  rot(2, pose[2], rz, drz);
-  double tc[16], tmc[16];
-  identity(tc);
-  identity(tmc);
-  for (int a = 0; a < 3; ++a) {
-    tc[4 * a + 3] = c[a] + pose[3 + a];
-    tmc[4 * a + 3] = -c[a];
-  }
-  auto chain = [&](const double* a, const double* b, const double* cc, double* o) {
-    double t1[16], t2[16];
-    rvk_mat4_mul(a, b, t1);
-    rvk_mat4_mul(t1, cc, t2);
-    double t3[16];
-    rvk_mat4_mul(tc, t2, t3);
-    rvk_mat4_mul(t3, tmc, o);
-  };
-  chain(rz, ry, rx, out);
-  if (dout) {
-    chain(rz, ry, drx, dout + 0);
-    chain(rz, dry, rx, dout + 16);
-    chain(drz, ry, rx, dout + 32);
-    // translation: d/dt_a of T(c+t) R T(-c) = E_a3 (R T(-c) has last row e4)
-    for (int a = 0; a < 3; ++a) {
-      double* d = dout + 16 * (3 + a);
-      std::memset(d, 0, 16 * sizeof(double));
-      d[4 * a + 3] = 1.0;
-    }
-  }
-}
-
-int rvk_posed_view(const rvk_camera* cam, const double pose[6], const double center[3], const double origin[3],
-                   const double spacing[3], int width, int height, int flipped, rvk_view* view, float* dm_out) {
-  if (!cam || !pose || !center || !view || width <= 0 || height <= 0) return RVK_ERR_INVALID_ARGUMENT;
-  double u[16], du[96];
-  rvk_pose_matrix(pose, center, u, du);
-  double a[16], m[16];
-  int rc = rvk_compute_matrices(cam, u, origin, spacing, (double)width / (double)height, m, a);
-  if (rc) return rc;
-  std::memset(view, 0, sizeof(*view));
-  rvk_prebake_view_to_voxels(m, view->view_to_voxels);
-  view->width = width;
-  view->height = height;
-  view->flipped = flipped;
-  if (dm_out) {
-    // d(A^-1) = -A^-1 dA A^-1 with dA = P V dU Vox
-    double p[16], v[16], pv[16], vox[16];
-    projection_matrix(cam, (double)width / (double)height, p);
-    view_matrix(cam, v);
-    rvk_mat4_mul(p, v, pv);
-    voxels_to_world(nullptr, origin, spacing, vox);
-    for (int i = 0; i < 6; ++i) {
-      double t1[16], da[16], t2[16], dm[16];
-      rvk_mat4_mul(pv, du + 16 * i, t1);
-      rvk_mat4_mul(t1, vox, da);
-      rvk_mat4_mul(m, da, t2);
-      rvk_mat4_mul(t2, m, dm);
-      for (int k = 0; k < 16; ++k) dm[k] = -dm[k];
-      // the pre-bake is linear, so it maps derivatives the same way
-      float* o = dm_out + 16 * i;
-      for (int k = 0; k < 16; ++k) o[k] = (float)dm[k];
-      for (int r = 0; r < 4; ++r) {
-        o[4 * r + 3] += o[4 * r + 0] - o[4 * r + 1];
-        o[4 * r + 0] *= -2.0f;
-        o[4 * r + 1] *= 2.0f;
-      }
-    }
-  }
   return RVK_OK;
 }
 

@@ -7,6 +7,7 @@
 
 #include "rvk.h"
 #include "rvk_device.h"
+#include "rvk_geometry.h"
 
 struct rvk_ctx {
   int device = 0;
@@ -145,6 +146,9 @@ float min_spacing(const rvk_volume* v);
 int check_view(const rvk_view* v);
 int check_frame(const rvk_volume* v, int frame);
 int ensure_scratch(rvk_ctx* ctx, size_t bytes);
+// Camera transform P V (rvk_geometry.cpp) and voxel -> world matrix.
+int camera_pv(const rvk_camera* cam, double aspect, double pv[16]);
+void voxels_matrix(const double origin[3], const double spacing[3], double vox[16]);
 
 inline int check_march(const rvk_march_params* m) {
   if (!m) return fail(RVK_ERR_INVALID_ARGUMENT, "march params are NULL");

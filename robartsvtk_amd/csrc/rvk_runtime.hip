@@ -371,6 +371,27 @@ static int default_layout(const int32_t dims[3], int dtype) {
 
 using namespace rvk;
 
+// ---------------------------------------------------------------------------
+// Streaming-read probe: every 16-byte word of a buffer read once, coalesced,
+// with a grid that fills the device.  The loads feed an xor that is stored
+// only on a match no real buffer produces, so they are not dropped.
+namespace rvk {
+__global__ __launch_bounds__(256) void stream_read_kernel(const uint4* __restrict__ p, size_t n, uint32_t* sink) {
+  uint32_t acc = 0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const uint4 a = p[i], b = p[i + stride], c = p[i + 2 * stride], d = p[i + 3 * stride];
+    acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^ d.z ^ d.w;
+  }
+  for (; i < n; i += stride) {
+    const uint4 a = p[i];
+    acc ^= a.x ^ a.y ^ a.z ^ a.w;
+  }
+  if (acc == 0x9e3779b9u && sink) sink[0] = acc;
+}
+}  // namespace rvk
+
 extern "C" {
 
 const char* rvk_last_error_string(void) { return g_last_error.c_str(); }
@@ -839,6 +860,21 @@ int rvk_tf2d_destroy(rvk_tf2d* t) {
   if (t->inex_col) hipFree(t->inex_col);
   if (t->col) hipFree(t->col);
   delete t;
+  return RVK_OK;
+}
+
+int rvk_stream_read(rvk_ctx* ctx, const void* buf, size_t bytes, uint32_t* sink) {
+  if (!ctx || !buf || bytes < 16 || (bytes & 15) || ((uintptr_t)buf & 15))
+    return fail(RVK_ERR_INVALID_ARGUMENT, "need a 16-byte aligned buffer of a multiple of 16 bytes");
+  RVK_HIP(hipSetDevice(ctx->device));
+  int n_cu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) n_cu = prop.multiProcessorCount;
+  const size_t n = bytes / 16;
+  const size_t want = (n + 255) / 256;
+  const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(want, (size_t)n_cu * 8));
+  hipLaunchKernelGGL(stream_read_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4*)buf, n, sink);
+  RVK_HIP(hipGetLastError());
   return RVK_OK;
 }
 

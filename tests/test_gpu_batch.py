@@ -33,10 +33,12 @@ def ncc64(img, tgt):
 
 
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("skip", [0, 1])
 @pytest.mark.parametrize("schedule", [abi.SCHED_GRID, abi.SCHED_PERSISTENT])
-def test_batch_matches_single_view_renders(gpu_ctx, schedule):
+def test_batch_matches_single_view_renders(gpu_ctx, schedule, skip):
     torch = _torch()
     s = configs.scene_c4(scale=0.125, n_poses=24)
+    s.march = dict(s.march, skip_rule=skip)
     vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
     mp, dp = configs.march_params(s.march), configs.drr_params(s.drr)
     plan = abi.DrrPlan(gpu_ctx, vol, s.views, mp, dp)
@@ -164,16 +166,25 @@ def _registration_scene(n_views=3, scale=0.125):
     return s, tg
 
 
+def _view_size(s, v):
+    if "sizes" in s.extra:
+        return s.extra["sizes"][v], s.extra["sizes"][v]
+    return s.extra["img"], s.extra["img"]
+
+
 def _fd_pose_gradient(s, tg, v, h=1e-3):
-    cam, c, img = s.extra["cameras"][v], s.extra["center"], s.extra["img"]
+    """FP64 central differences of 1 - NCC(DRR(pose), target) at view v's
+    pose, with the pose matrices formed in double precision."""
+    cam, c = s.extra["cameras"][v], s.extra["center"]
+    W, Hh = _view_size(s, v)
     g = np.zeros(6)
     for i in range(6):
         vals = []
         for sgn in (1.0, -1.0):
             pose = np.array(s.extra["poses"][v], float)
             pose[i] += sgn * h
-            vd = {"m": np.zeros(16, np.float32), "W": img, "H": img,
-                  "m64": baked64(cam, pose, c, img, img, s.spacing)}
+            vd = {"m": np.zeros(16, np.float32), "W": W, "H": Hh,
+                  "m64": baked64(cam, pose, c, W, Hh, s.spacing)}
             I = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f64")
             vals.append(1.0 - ncc64(I, tg[v]))
         g[i] = (vals[0] - vals[1]) / (2 * h)
@@ -341,3 +352,130 @@ def test_register_and_volume_gradient_identical_across_cell_layouts(gpu_ctx):
         vol.close()
     for a, b in zip(*res):
         assert torch.equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+def test_skip_rule_pose_gradient_vs_fp64_fd(gpu_ctx):
+    """The adjoint of the skip-rule march (the batched kernels' SKIP
+    instances): the skip rule changes which samples are taken only where the
+    march leaves and re-enters contributing values, so between such changes
+    the accumulated samples are fixed and the pose gradient is that of their
+    sum.  Two poses of a reduced c4 scene (int16 HU volume), against FP64
+    central differences of the skip-rule march at h = 1e-6."""
+    torch = _torch()
+    s = configs.scene_c4(scale=0.125, n_poses=2, with_derivatives=True)
+    s.march = dict(s.march, skip_rule=1)
+    c = s.extra["center"]
+    rng = np.random.default_rng(41)
+    gt = s.extra["poses"] + np.concatenate([np.radians(rng.uniform(-2, 2, (2, 3))), rng.uniform(-4, 4, (2, 3))], 1)
+    tg = []
+    for v in range(2):
+        W = s.extra["sizes"][v]
+        vd = {"m": np.zeros(16, np.float32), "W": W, "H": W,
+              "m64": baked64(s.extra["cameras"][v], gt[v], c, W, W, s.spacing)}
+        t = orc.drr(s.volume, s.spacing, vd, s.march, s.drr, "f64")
+        tg.append((t + rng.normal(0.0, 0.01, t.shape)).astype(np.float32))
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    plan = abi.DrrPlan(gpu_ctx, vol, s.views, configs.march_params(s.march), configs.drr_params(s.drr),
+                       dview=s.extra["dview"])
+    tgt = torch.from_numpy(np.concatenate([t.ravel() for t in tg])).cuda()
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    loss = torch.empty(2, dtype=torch.float32, device="cuda")
+    grad = torch.empty(12, dtype=torch.float32, device="cuda")
+    plan.register(out, tgt, loss, grad)
+    # and the explicit backward of the skip-rule march agrees with the fused step
+    dldi = torch.empty_like(out)
+    plan.ncc(out, tgt, torch.empty_like(loss), dldi)
+    grad2 = torch.empty_like(grad)
+    plan.backward(out, dldi, grad_pose=grad2)
+    torch.cuda.synchronize()
+    torch.testing.assert_close(grad2, grad, rtol=1e-5, atol=1e-7)
+    grad = grad.cpu().numpy().reshape(2, 6)
+    for v in range(2):
+        fd6 = _fd_pose_gradient(s, tg, v, 1e-6)
+        e6 = np.linalg.norm(grad[v] - fd6) / np.linalg.norm(fd6)
+        print(f"skip-rule view {v}: grad {grad[v]}, fd(1e-6) {fd6}, rel err {e6:.2e}")
+        assert e6 <= 1e-3, (v, e6)
+    plan.close()
+    vol.close()
+
+
+def _ortho_view(W, H, x0, a, z0, b, y_from, length):
+    """Parallel rays along +y: pixel (x, y) starts at (x0 + a vx, y_from,
+    z0 + b vy) with vx = 1 - x/W, vy = y/H, and runs `length` voxels."""
+    v = abi.View()
+    v.view_to_voxels[:] = [a, 0, 0, x0, 0, 0, length, y_from, 0, b, 0, z0, 0, 0, 0, 1]
+    v.width, v.height = W, H
+    return v
+
+
+def test_volume_gradient_per_voxel_edges_and_keyhole(gpu_ctx):
+    """d loss / d volume of a linear loss, voxel by voxel, against FP64
+    central differences (the FP64 march replays the FP32 step counts), on a
+    small strictly positive volume: so no sample switches between
+    contributing and not, and the loss is smooth in every voxel.  Views:
+    (1) parallel rays through x in [0.1, 0.45] voxels, i.e. cell coordinates
+    below 1, where the clamped edge cell holds voxel 0 twice; (2) a cone-beam
+    view with keyhole planes, whose window's samples are not accumulated."""
+    torch = _torch()
+    dims = (11, 10, 9)
+    rng = np.random.default_rng(23)
+    mu = (0.02 + 0.03 * rng.random(dims[::-1])).astype(np.float32)
+    spacing = (1.0, 1.2, 0.9)
+    march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": 0, "precise": 1}
+    drr = {"intercept": 0.0, "slope": 1.0, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L, "ref_termination": 0}
+    views = [_ortho_view(8, 6, 0.05, 0.4, 2.3, 5.0, -2.0, dims[1] + 4.0)]
+    c = configs.volume_center(dims, spacing)
+    cam = configs.drr_camera(c, 0.7, 12, sod=60.0, sid=100.0, fov=40.0, half_extent=14.0)
+    key = abi.compute_view(cam, 12, 12, spacing=spacing)
+    key = _with_keyhole(key, dims[0])
+    views.append(key)
+    vol = abi.Volume.from_array(gpu_ctx, mu, spacing)
+    plan = abi.DrrPlan(gpu_ctx, vol, views, configs.march_params(march), configs.drr_params(drr))
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    plan.render(out)
+    w = torch.from_numpy(rng.standard_normal(plan.total).astype(np.float32)).cuda()
+    gvol = torch.zeros(mu.size, dtype=torch.float32, device="cuda")
+    plan.backward(out, w, grad_volume=gvol)
+    torch.cuda.synchronize()
+    g = gvol.cpu().numpy().reshape(mu.shape).astype(np.float64)
+    wv = w.cpu().numpy().astype(np.float64)
+    vds = [configs.view_dict(v) for v in views]
+    m64 = dict(march, f32_control=1)
+    # the keyhole window does exclude samples of view 2
+    full = orc.drr(mu, spacing, dict(vds[1], keyhole=[]), march, drr, "f32")
+    assert not np.array_equal(full, orc.drr(mu, spacing, vds[1], march, drr, "f32"))
+
+    def loss(vol):
+        return sum(float(orc.drr(vol, spacing, vds[v], m64, drr, "f64", nthreads=1).ravel() @
+                         wv[plan.offsets[v]:plan.offsets[v + 1]]) for v in range(len(views)))
+
+    h = 1e-3
+    fd = np.zeros_like(g)
+    for idx in np.ndindex(*mu.shape):
+        up, dn = mu.copy(), mu.copy()
+        up[idx] += h
+        dn[idx] -= h
+        fd[idx] = (loss(up) - loss(dn)) / (2 * h)
+    err = np.abs(g - fd)
+    scale = np.abs(fd).max()
+    print(f"per-voxel volume gradient: max |g - fd| {err.max():.3e} of max |fd| {scale:.3e}; "
+          f"x = 0 slab max |fd| {np.abs(fd[:, :, 0]).max():.3e}")
+    assert np.abs(fd[:, :, 0]).max() > 0.1 * scale  # the edge rays weigh on voxel x = 0
+    np.testing.assert_allclose(g, fd, rtol=1e-3, atol=1e-4 * scale)
+    plan.close()
+    vol.close()
+
+
+def test_pose_backward_needs_dview(gpu_ctx):
+    torch = _torch()
+    s = configs.scene_c4(scale=0.125, n_poses=2)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    plan = abi.DrrPlan(gpu_ctx, vol, s.views, configs.march_params(s.march), configs.drr_params(s.drr))
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    plan.render(out)
+    with pytest.raises(abi.RvkError) as e:
+        plan.backward(out, torch.ones_like(out), grad_pose=torch.empty(12, dtype=torch.float32, device="cuda"))
+    assert e.value.status == 5
+    plan.close()
+    vol.close()

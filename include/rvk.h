@@ -192,11 +192,14 @@ int rvk_volume_destroy(rvk_volume* vol);
 /* Cell order of the device layout (no reference counterpart: the reference
  * keeps a float cudaArray whose tiling the driver picks).  Each trilinear
  * 2x2x2 footprint is stored whole; RVK_CELLS_PLAIN orders the cells x-fastest,
- * RVK_CELLS_BRICK4 groups them in 4x4x4 bricks.  The default is BRICK4 when
- * the cells take more than 1 GiB (or PLAIN would exceed its limit of 2^24
- * cell rows, (ny + 2)(nz + 2)), PLAIN below; set_layout must come before
- * the first rvk_volume_set_frame.  Renders give identical results in both. */
-enum { RVK_CELLS_PLAIN = 0, RVK_CELLS_BRICK4 = 1 };
+ * RVK_CELLS_BRICK4 groups them in 4x4x4 bricks.  RVK_CELLS_PAIR8 stores, per
+ * 8x8x8 brick of cells, each x-pair of voxels its cells use once (9 x 9 rows
+ * of 8 pairs: 2.53x the native voxel bytes instead of 8x) and reads a cell as
+ * four pairs.  The default is BRICK4 when the cells take more than 1 GiB (or
+ * PLAIN would exceed its limit of 2^24 cell rows, (ny + 2)(nz + 2)), PLAIN
+ * below; set_layout must come before the first rvk_volume_set_frame.
+ * Renders give identical results in every layout. */
+enum { RVK_CELLS_PLAIN = 0, RVK_CELLS_BRICK4 = 1, RVK_CELLS_PAIR8 = 2 };
 int rvk_volume_set_layout(rvk_volume* vol, int layout);
 int rvk_volume_get_layout(const rvk_volume* vol, int* layout);
 int rvk_volume_dims(const rvk_volume* vol, int32_t dims[3], double spacing[3], int* dtype, int* n_frames);

@@ -422,7 +422,7 @@ def count_ray_steps(ctx: Context, vol: Volume, view: View, march: MarchParams) -
 
 
 SCHED_GRID, SCHED_PERSISTENT = 1, 2
-CELLS_PLAIN, CELLS_BRICK4 = 0, 1
+CELLS_PLAIN, CELLS_BRICK4, CELLS_PAIR8 = 0, 1, 2
 
 
 class DrrPlan:

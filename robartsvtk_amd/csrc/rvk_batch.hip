@@ -1090,6 +1090,9 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     if (p->vol->layout == kCellLayoutBrick4)
       hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutBrick4>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
                          s, va);
+    else if (p->vol->layout == kCellLayoutPair8)
+      hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPair8>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
+                         s, va);
     else
       hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPlain>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s,
                          va);

@@ -126,6 +126,35 @@ template <int LOG2> struct CellOrder {
 constexpr int kCellLayoutPlain = 0;
 constexpr int kCellLayoutBrick4 = 2;
 
+// X-pair bricks (layout 3): the cells of an 8x8x8 cell brick are not stored
+// whole; the brick keeps every x-pair (v[x], v[x+1]) its cells use once, as
+// 9 z-slabs x 9 y-rows x 8 pairs (the 9 voxel rows 8b-1 .. 8b+7 a brick's
+// cells reach per axis, clamped at the volume edges).  A cell's 4 pairs are
+// then pairs p, p + 8 (next row), p + 72 (next slab) and p + 80 of its
+// brick: one address and four loads with constant offsets, and 2.53x the
+// native voxel bytes instead of the 8x of whole cells.
+constexpr int kCellLayoutPair8 = 3;
+constexpr int kPairRow = 8;                // pairs per brick row
+constexpr int kPairSlab = 9 * kPairRow;    // pairs per brick slab
+constexpr int kPairBrick = 9 * kPairSlab;  // 648 pairs per brick
+
+template <> struct CellOrder<kCellLayoutPair8> {
+  static constexpr int kLog2 = kCellLayoutPair8;
+  // index() returns the cell's first pair (pair units: 2 voxels)
+  static __device__ __forceinline__ uint32_t index(const VolGeom& g, int cx, int cy, int cz) {
+    // brick (cx >> 3, cy >> 3, cz >> 3) of a grid cys x crows x ..., then
+    // slab, row and pair inside it; bricks and in-brick offsets stay < 2^24
+    const uint32_t b = __umul24(__umul24((uint32_t)(cz >> 3), g.crows) + (uint32_t)(cy >> 3), g.cys) +
+                       (uint32_t)(cx >> 3);
+    const uint32_t in = __umul24((uint32_t)(cz & 7), (uint32_t)kPairSlab) + (uint32_t)((cy & 7) * kPairRow) +
+                        (uint32_t)(cx & 7);
+    return __umul24(b, (uint32_t)kPairBrick) + in;
+  }
+};
+// Voxels per unit of CellOrder<L>::index(): a whole cell (8) or a pair (2).
+template <int L> struct CellStride { static constexpr int k = 8; };
+template <> struct CellStride<kCellLayoutPair8> { static constexpr int k = 2; };
+
 // Range bricks: kRangeEdge^3 cells (cells [8b, 8b + 8) per axis, i.e. the
 // voxels [8b - 1, 8b + 7] their corners reach).
 constexpr int kRangeLog2 = 3;
@@ -149,10 +178,20 @@ template <typename T, int LOG2, bool DIFF = false> struct Cells {
 // as floats (`unpack`, in the order dx + 2 dy + 4 dz).  `unpack_ad` gives the
 // four x-pairs as a = v(dx=0) and d = v(dx=1) - v(dx=0) for either encoding.
 template <typename VT> struct CellLoad;
+// raw_pairs(d, p) assembles the same Raw from the four x-pairs of an x-pair
+// brick cell (pairs p, p + 8, p + 72, p + 80).
+template <typename P> __device__ __forceinline__ P pair_at(const void* d, uint32_t p) {
+  return reinterpret_cast<const P*>(d)[p];
+}
 template <> struct CellLoad<uint8_t> {
   using Raw = uint2;
   static __device__ __forceinline__ Raw raw(const uint8_t* __restrict__ d, uint32_t c) {
     return *reinterpret_cast<const uint2*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ Raw raw_pairs(const uint8_t* __restrict__ d, uint32_t p) {
+    const uint32_t a = pair_at<uint16_t>(d, p), b = pair_at<uint16_t>(d, p + kPairRow);
+    const uint32_t c = pair_at<uint16_t>(d, p + kPairSlab), e = pair_at<uint16_t>(d, p + kPairSlab + kPairRow);
+    return Raw{a | (b << 16), c | (e << 16)};
   }
   static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
 #pragma unroll
@@ -176,6 +215,11 @@ template <> struct CellLoad<int8_t> {
   static __device__ __forceinline__ Raw raw(const int8_t* __restrict__ d, uint32_t c) {
     return *reinterpret_cast<const int2*>(d + (size_t)c * 8);
   }
+  static __device__ __forceinline__ Raw raw_pairs(const int8_t* __restrict__ d, uint32_t p) {
+    const uint32_t a = pair_at<uint16_t>(d, p), b = pair_at<uint16_t>(d, p + kPairRow);
+    const uint32_t c = pair_at<uint16_t>(d, p + kPairSlab), e = pair_at<uint16_t>(d, p + kPairSlab + kPairRow);
+    return Raw{(int)(a | (b << 16)), (int)(c | (e << 16))};
+  }
   static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -193,10 +237,18 @@ template <> struct CellLoad<int8_t> {
     }
   }
 };
+// 16-bit voxels: a pair is one 32-bit word.
+__device__ __forceinline__ uint4 raw_pair_words(const void* d, uint32_t p) {
+  return uint4{pair_at<uint32_t>(d, p), pair_at<uint32_t>(d, p + kPairRow), pair_at<uint32_t>(d, p + kPairSlab),
+               pair_at<uint32_t>(d, p + kPairSlab + kPairRow)};
+}
 template <> struct CellLoad<uint16_t> {
   using Raw = uint4;
   static __device__ __forceinline__ Raw raw(const uint16_t* __restrict__ d, uint32_t c) {
     return *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ Raw raw_pairs(const uint16_t* __restrict__ d, uint32_t p) {
+    return raw_pair_words(d, p);
   }
   static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
     const uint32_t q[4] = {w.x, w.y, w.z, w.w};
@@ -220,6 +272,10 @@ template <> struct CellLoad<int16_t> {
   using Raw = int4;
   static __device__ __forceinline__ Raw raw(const int16_t* __restrict__ d, uint32_t c) {
     return *reinterpret_cast<const int4*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ Raw raw_pairs(const int16_t* __restrict__ d, uint32_t p) {
+    const uint4 w = raw_pair_words(d, p);
+    return Raw{(int)w.x, (int)w.y, (int)w.z, (int)w.w};
   }
   static __device__ __forceinline__ void unpack(Raw w, float v[8]) {
     const int q[4] = {w.x, w.y, w.z, w.w};
@@ -245,6 +301,15 @@ template <typename W, typename V4> struct CellLoad32 {
   static __device__ __forceinline__ Raw raw(const W* __restrict__ d, uint32_t c) {
     const V4* p = reinterpret_cast<const V4*>(d + (size_t)c * 8);
     return Raw{p[0], p[1]};
+  }
+  static __device__ __forceinline__ Raw raw_pairs(const W* __restrict__ d, uint32_t p) {
+    const uint2 a = pair_at<uint2>(d, p), b = pair_at<uint2>(d, p + kPairRow);
+    const uint2 c = pair_at<uint2>(d, p + kPairSlab), e = pair_at<uint2>(d, p + kPairSlab + kPairRow);
+    using E = decltype(V4{}.x);
+    return Raw{V4{__builtin_bit_cast(E, a.x), __builtin_bit_cast(E, a.y), __builtin_bit_cast(E, b.x),
+                  __builtin_bit_cast(E, b.y)},
+               V4{__builtin_bit_cast(E, c.x), __builtin_bit_cast(E, c.y), __builtin_bit_cast(E, e.x),
+                  __builtin_bit_cast(E, e.y)}};
   }
   static __device__ __forceinline__ void unpack(const Raw& w, float v[8]) {
     v[0] = (float)w.a.x; v[1] = (float)w.a.y; v[2] = (float)w.a.z; v[3] = (float)w.a.w;
@@ -275,6 +340,9 @@ template <> struct CellLoad<Half16> {
   using Raw = uint4;
   static __device__ __forceinline__ Raw raw(const Half16* __restrict__ d, uint32_t c) {
     return *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
+  }
+  static __device__ __forceinline__ Raw raw_pairs(const Half16* __restrict__ d, uint32_t p) {
+    return raw_pair_words(d, p);
   }
   static __device__ __forceinline__ float base(uint32_t q) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)(q & 0xffffu));
@@ -310,6 +378,9 @@ template <> struct CellLoad<HalfPair> {
   static __device__ __forceinline__ Raw raw(const HalfPair* __restrict__ d, uint32_t c) {
     return *reinterpret_cast<const uint4*>(d + (size_t)c * 8);
   }
+  static __device__ __forceinline__ Raw raw_pairs(const HalfPair* __restrict__ d, uint32_t p) {
+    return raw_pair_words(d, p);
+  }
   static __device__ __forceinline__ float lo(uint32_t q) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)(q & 0xffffu));
   }
@@ -339,10 +410,17 @@ template <typename CV> __device__ __forceinline__ CV cells_of(const void* p) {
   return CV{static_cast<const typename CV::value_type*>(p)};
 }
 
+// The raw bits of the cell at CellOrder<L>::index() c.
+template <typename T, int L>
+__device__ __forceinline__ typename CellLoad<T>::Raw fetch_cell(const T* __restrict__ d, uint32_t c) {
+  if constexpr (L == kCellLayoutPair8) return CellLoad<T>::raw_pairs(d, c);
+  else return CellLoad<T>::raw(d, c);
+}
+
 // Voxel (i,j,k) (in range) = corner 000 of cell (i+1, j+1, k+1).
 template <typename T, int L, bool D>
 __device__ __forceinline__ float load_voxel(Cells<T, L, D> v, const VolGeom& g, int i, int j, int k) {
-  return (float)v.d[(size_t)CellOrder<L>::index(g, i + 1, j + 1, k + 1) * 8];
+  return (float)v.d[(size_t)CellOrder<L>::index(g, i + 1, j + 1, k + 1) * CellStride<L>::k];
 }
 
 // Cell and weights of tex3D(unnormalised, linear, clamp) at p, given the
@@ -381,7 +459,7 @@ template <bool CLAMP = true, typename T, int L, bool D>
 __device__ __forceinline__ float trilinear(Cells<T, L, D> vol, const VolGeom& g, float cx, float cy, float cz) {
   const CellPos c = cell_pos<L, CLAMP>(g, cx, cy, cz);
   float a[4], d[4];
-  CellLoad<T>::template unpack_ad<D>(CellLoad<T>::raw(vol.d, c.cell), a, d);
+  CellLoad<T>::template unpack_ad<D>(fetch_cell<T, L>(vol.d, c.cell), a, d);
   // lerp(a, b, t) = fma(t, b - a, a) with b - a = d
   const float c00 = fmaf(c.ax, d[0], a[0]), c10 = fmaf(c.ax, d[1], a[1]);
   const float c01 = fmaf(c.ax, d[2], a[2]), c11 = fmaf(c.ax, d[3], a[3]);
@@ -403,7 +481,7 @@ __device__ __forceinline__ float trilinear_vg(Cells<T, L, D> vol, const VolGeom&
                                               V3<float>& gr) {
   const CellPos c = cell_pos<L, CLAMP>(g, cx, cy, cz);
   float a[4], d[4];
-  CellLoad<T>::template unpack_ad<D>(CellLoad<T>::raw(vol.d, c.cell), a, d);
+  CellLoad<T>::template unpack_ad<D>(fetch_cell<T, L>(vol.d, c.cell), a, d);
   const float c00 = fmaf(c.ax, d[0], a[0]), c10 = fmaf(c.ax, d[1], a[1]);
   const float c01 = fmaf(c.ax, d[2], a[2]), c11 = fmaf(c.ax, d[3], a[3]);
   const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);

@@ -24,9 +24,10 @@ struct rvk_volume {
   double spacing[3] = {1, 1, 1};
   int dtype = RVK_U8;
   int n_frames = 0;
-  int layout = 0;                // cell order: kCellLayoutPlain or kCellLayoutBrick4
+  int layout = 0;                // cell order: kCellLayoutPlain, kCellLayoutBrick4 or kCellLayoutPair8
   int ncb[3] = {0, 0, 0};        // cell-brick grid: ceil((dims+2)/brick edge)
-  size_t n_cells = 0;            // cells incl. brick padding (each 8 voxels)
+  size_t n_cells = 0;            // storage in units of 8 voxels (whole cells incl. brick padding, or
+                                 // x-pair bricks: 162 units each)
   void* frames[RVK_MAX_FRAMES] = {};    // cell volumes, n_cells * 8 voxels
   uint8_t enc[RVK_MAX_FRAMES] = {};     // cell encoding of each frame: kEnc* (see Cells, Half16)
   uint8_t* gradmag[RVK_MAX_FRAMES] = {};  // u8 cell volumes
@@ -131,6 +132,7 @@ template <int L, bool HALF, typename F> int dispatch_cells_t(int dtype, int enc,
 }
 template <bool HALF = true, typename F> int dispatch_cells(int dtype, int layout, int enc, F&& f) {
   if (layout == kCellLayoutBrick4) return dispatch_cells_t<kCellLayoutBrick4, HALF>(dtype, enc, f);
+  if (layout == kCellLayoutPair8) return dispatch_cells_t<kCellLayoutPair8, HALF>(dtype, enc, f);
   return dispatch_cells_t<kCellLayoutPlain, HALF>(dtype, enc, f);
 }
 // The cells of one frame of a volume.

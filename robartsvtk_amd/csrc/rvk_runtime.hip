@@ -59,6 +59,13 @@ VolGeom vol_geom(const rvk_volume* v) {
     g.bounds[2 * a] = 0.0f;
     g.bounds[2 * a + 1] = (float)v->dims[a] - 1.0f;
   }
+  if (v->layout == kCellLayoutPair8) {
+    // x-pair bricks: the brick grid's row length and rows per slab
+    g.cys = (uint32_t)v->ncb[0];
+    g.crows = (uint32_t)v->ncb[1];
+    g.czs = (uint32_t)v->ncb[0] * (uint32_t)v->ncb[1];
+    return g;
+  }
   const uint32_t bc = 1u << (3 * v->layout);  // cells per brick
   g.cys = bc * (uint32_t)v->ncb[0];
   g.czs = bc * (uint32_t)v->ncb[0] * (uint32_t)v->ncb[1];
@@ -182,6 +189,36 @@ __global__ __launch_bounds__(256) void cell_relayout_kernel(const VT* __restrict
   }
 }
 
+// Linear -> x-pair bricks.  One thread per stored pair: brick (bx, by, bz),
+// slab lz and row ly (voxel rows 8b - 1 + l, clamped) and pair lx, holding
+// voxels x = 8 bx + lx - 1 and x + 1 (clamped), encoded like a cell's pairs.
+template <typename VT, bool DIFF = false, int HALF = 0>
+__global__ __launch_bounds__(256) void pair_relayout_kernel(const VT* __restrict__ src, VT* __restrict__ dst, int nx,
+                                                            int ny, int nz, int nbx, int nby, long long n_pairs) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  const long long b = p / kPairBrick;
+  const int r = (int)(p % kPairBrick);
+  const int lz = r / kPairSlab, ly = (r / kPairRow) % 9, lx = r % kPairRow;
+  const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+  const int x = bx * 8 + lx - 1;
+  const int j = min(max(by * 8 + ly - 1, 0), ny - 1);
+  const int k = min(max(bz * 8 + lz - 1, 0), nz - 1);
+  const long long row = (long long)j * nx + (long long)k * nx * ny;
+  VT a = src[row + min(max(x, 0), nx - 1)];
+  VT c = src[row + min(max(x + 1, 0), nx - 1)];
+  if (DIFF) {
+    if constexpr (sizeof(VT) == 2) c = (VT)(uint16_t)((int)c - (int)a);
+    else c = c - a;
+  }
+  if constexpr (HALF > 0 && sizeof(VT) == 2) {
+    a = (VT)__builtin_bit_cast(uint16_t, (_Float16)(float)(int)a);
+    if constexpr (HALF > 1) c = (VT)__builtin_bit_cast(uint16_t, (_Float16)(float)(int)(int16_t)(uint16_t)c);
+  }
+  dst[2 * p] = a;
+  dst[2 * p + 1] = c;
+}
+
 // Gradient magnitude by central differences at voxel centres, the value the
 // reference's on-the-fly gradient takes there (tex3D at p +- 1 with clamp,
 // times 0.5/spacing; V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:263-271),
@@ -209,10 +246,17 @@ __global__ __launch_bounds__(256) void gradmag_kernel(CV vol, uint8_t* __restric
 
 template <typename VT, int L, bool DIFF = false, int HALF = 0>
 static int launch_relayout(rvk_volume* v, const void* dsrc, void* dst) {
-  const long long total = (long long)v->n_cells;
-  const int blocks = (int)((total + 255) / 256);
-  hipLaunchKernelGGL((cell_relayout_kernel<VT, L, DIFF, HALF>), dim3(blocks), dim3(256), 0, v->ctx->stream,
-                     (const VT*)dsrc, (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
+  if constexpr (L == kCellLayoutPair8) {
+    const long long total = (long long)v->n_cells * 4;  // pairs: 8 voxels per storage unit
+    const int blocks = (int)((total + 255) / 256);
+    hipLaunchKernelGGL((pair_relayout_kernel<VT, DIFF, HALF>), dim3(blocks), dim3(256), 0, v->ctx->stream,
+                       (const VT*)dsrc, (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
+  } else {
+    const long long total = (long long)v->n_cells;
+    const int blocks = (int)((total + 255) / 256);
+    hipLaunchKernelGGL((cell_relayout_kernel<VT, L, DIFF, HALF>), dim3(blocks), dim3(256), 0, v->ctx->stream,
+                       (const VT*)dsrc, (VT*)dst, v->dims[0], v->dims[1], v->dims[2], v->ncb[0], v->ncb[1], total);
+  }
   RVK_HIP(hipGetLastError());
   return RVK_OK;
 }
@@ -345,6 +389,14 @@ static int size_cells(rvk_volume* v) {
   const int edge = 1 << v->layout;
   for (int a = 0; a < 3; ++a) v->ncb[a] = (v->dims[a] + 2 + edge - 1) / edge;
   for (int a = 0; a < 3; ++a) v->nrb[a] = (v->dims[a] + 2 + kRangeEdge - 1) / kRangeEdge;
+  if (v->layout == kCellLayoutPair8) {
+    // storage units of 8 voxels: a brick's 648 pairs are 162 of them
+    const size_t nb = (size_t)v->ncb[0] * v->ncb[1] * v->ncb[2];
+    v->n_cells = nb * (kPairBrick / 4);
+    if (nb * kPairBrick > (size_t)0xffffffffu || (size_t)v->ncb[0] * v->ncb[1] >= ((size_t)1 << 24))
+      return fail(RVK_ERR_INVALID_ARGUMENT, "volume too large for 32-bit x-pair indices");
+    return RVK_OK;
+  }
   const size_t bc = (size_t)1 << (3 * v->layout);
   v->n_cells = bc * v->ncb[0] * v->ncb[1] * v->ncb[2];
   const size_t slab = bc * v->ncb[0] * v->ncb[1];
@@ -550,12 +602,14 @@ int rvk_volume_create(rvk_ctx* ctx, const int32_t dims[3], const double spacing[
 
 int rvk_volume_set_layout(rvk_volume* v, int layout) {
   if (!v) return fail(RVK_ERR_INVALID_ARGUMENT, "volume is NULL");
-  if (layout != RVK_CELLS_PLAIN && layout != RVK_CELLS_BRICK4)
+  if (layout != RVK_CELLS_PLAIN && layout != RVK_CELLS_BRICK4 && layout != RVK_CELLS_PAIR8)
     return fail(RVK_ERR_INVALID_ARGUMENT, "unknown cell layout %d", layout);
   for (int f = 0; f < RVK_MAX_FRAMES; ++f)
     if (v->frames[f]) return fail(RVK_ERR_INVALID_ARGUMENT, "the cell layout is fixed once a frame has data");
   const int old = v->layout;
-  v->layout = layout == RVK_CELLS_BRICK4 ? kCellLayoutBrick4 : kCellLayoutPlain;
+  v->layout = layout == RVK_CELLS_BRICK4 ? kCellLayoutBrick4
+              : layout == RVK_CELLS_PAIR8 ? kCellLayoutPair8
+                                          : kCellLayoutPlain;
   const int rc = size_cells(v);
   if (rc != RVK_OK) {
     v->layout = old;
@@ -566,7 +620,9 @@ int rvk_volume_set_layout(rvk_volume* v, int layout) {
 
 int rvk_volume_get_layout(const rvk_volume* v, int* layout) {
   if (!v || !layout) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
-  *layout = v->layout == kCellLayoutBrick4 ? RVK_CELLS_BRICK4 : RVK_CELLS_PLAIN;
+  *layout = v->layout == kCellLayoutBrick4 ? RVK_CELLS_BRICK4
+            : v->layout == kCellLayoutPair8 ? RVK_CELLS_PAIR8
+                                            : RVK_CELLS_PLAIN;
   return RVK_OK;
 }
 

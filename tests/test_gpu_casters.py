@@ -195,7 +195,7 @@ def _sparse_volume(dtype, sign, dims=(61, 45, 53), seed=11):
 
 
 @pytest.mark.parametrize("dtype,sign", [("f32", 1), ("f32", -1), ("f32c", 1), ("i16", 1), ("i16", -1), ("u8", -1)])
-@pytest.mark.parametrize("layout", [abi.CELLS_PLAIN, abi.CELLS_BRICK4])
+@pytest.mark.parametrize("layout", [abi.CELLS_PLAIN, abi.CELLS_BRICK4, abi.CELLS_PAIR8])
 def test_drr_sparse_volumes(gpu_ctx, dtype, sign, layout):
     """Mostly empty volumes: the batched precise march passes over runs of
     samples that cannot contribute (or that contribute one constant alpha)
@@ -504,7 +504,7 @@ def test_plain_march_sample_count(gpu_ctx, which):
 def test_cell_layouts_render_identically(gpu_ctx):
     import torch
     outs = {}
-    for layout in (abi.CELLS_PLAIN, abi.CELLS_BRICK4):
+    for layout in (abi.CELLS_PLAIN, abi.CELLS_BRICK4, abi.CELLS_PAIR8):
         s2 = configs.scene_c2(scale=0.25)
         v2 = abi.Volume.from_array(gpu_ctx, s2.volume, s2.spacing, layout=layout)
         assert v2.layout() == layout
@@ -520,8 +520,9 @@ def test_cell_layouts_render_identically(gpu_ctx):
         outs[layout] = (d, c1, c3, c3b)
         for v in (v1, v2, v3):
             v.close()
-    for a, b in zip(outs[abi.CELLS_PLAIN], outs[abi.CELLS_BRICK4]):
-        assert np.array_equal(a, b)
+    for layout in (abi.CELLS_BRICK4, abi.CELLS_PAIR8):
+        for a, b in zip(outs[abi.CELLS_PLAIN], outs[layout]):
+            assert np.array_equal(a, b), layout
     assert outs[abi.CELLS_PLAIN][2][..., 3].max() > 100
 
 

@@ -264,6 +264,17 @@ int rvk_render_drr(rvk_ctx* ctx, const rvk_volume* vol, int frame, const rvk_vie
  * past the last row/column are clamped (the reference reads past them). */
 int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
                    const float* min_depth, const float* max_depth, uint8_t* rgba);
+/* rvk_post_shade without any allocation, for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, hosts that
+ * forbid it).  No counterpart in the reference.  Same arguments and results
+ * as rvk_post_shade; the per-call staging buffer of the normalised depth is
+ * instead a fixed device array of RVK_POST_SHADE_NOALLOC_MAX_PIXELS floats
+ * (16 MiB) per GPU, held until the process ends; larger images fail with
+ * RVK_ERR_INVALID_ARGUMENT.  Calls share that array, so calls that overlap on
+ * one GPU (different streams or threads) race. */
+#define RVK_POST_SHADE_NOALLOC_MAX_PIXELS 4194304
+int rvk_post_shade_noalloc(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
+                           const float* min_depth, const float* max_depth, uint8_t* rgba);
 /* The In/Ex-logic mapper's cel shade (V/CUDA_vtkCuda2DInExLogicVolumeMapper_renderAlgo.cuh:182-239):
  * squared depth differences, and pixels with zero alpha get
  * alpha = (signed char)(255 - 255 g) as a byte. */

@@ -126,6 +126,7 @@ _SIGS = {
                              C.POINTER(AuxOut)]),
     "rvk_post_shade": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
     "rvk_post_shade_inex": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
+    "rvk_post_shade_noalloc": (_I, [_P, _I, _I, C.POINTER(ShadeParams), _P, _P, _P, _P]),
     "rvk_count_ray_steps": (_I, [_P, _P, C.POINTER(View), C.POINTER(MarchParams), C.POINTER(C.c_int64)]),
     "rvk_stream_read": (_I, [_P, _P, C.c_size_t, _P]),
     "rvk_drr_plan_create": (_I, [_P, _P, _I, _I, C.POINTER(View), _P, C.POINTER(MarchParams), C.POINTER(DrrParams),

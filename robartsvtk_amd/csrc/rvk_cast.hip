@@ -12,6 +12,7 @@
 //   2D   V/CUDA_vtkCuda2DVolumeMapper_renderAlgo.cuh:30-470
 //   post V/CUDA_vtkCudaVolumeMapper_renderAlgo.cu:449-512
 #include <cmath>
+#include <mutex>
 
 #include "rvk_internal.h"
 #include "rvk_march.h"
@@ -587,6 +588,54 @@ static int post_shade(rvk_ctx* ctx, bool inex, int width, int height, const rvk_
 int rvk_post_shade(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
                    const float* min_depth, const float* max_depth, uint8_t* rgba) {
   return post_shade(ctx, false, width, height, shade, depth, min_depth, max_depth, rgba);
+}
+
+// The allocation-free post pass: its normalised-depth staging (the buffer
+// rvk_post_shade allocates per call) is a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code reaches a GPU's copy only
+// through the address hipGetSymbolAddress returns on that device (the
+// array's name on the host is a stand-in, not a device address).
+static __device__ __align__(256) float scratch[RVK_POST_SHADE_NOALLOC_MAX_PIXELS];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxScratchDevices] = {};
+
+// This device's copy of `scratch`, looked up on the first call on it (the
+// device must be current).
+int scratch_on(int device, float** out) {
+  if (device < 0 || device >= kMaxScratchDevices) return fail(RVK_ERR_INVALID_ARGUMENT, "device %d", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    float* p = nullptr;
+    RVK_HIP(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return RVK_OK;
+}
+}  // namespace
+
+int rvk_post_shade_noalloc(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,
+                           const float* min_depth, const float* max_depth, uint8_t* rgba) {
+  if (!ctx || !shade || !depth || !min_depth || !max_depth || !rgba)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (width <= 0 || height <= 0) return fail(RVK_ERR_INVALID_ARGUMENT, "bad image size");
+  if ((long long)width * height > (long long)RVK_POST_SHADE_NOALLOC_MAX_PIXELS)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "%d x %d pixels exceed the allocation-free post pass's %d", width, height,
+                RVK_POST_SHADE_NOALLOC_MAX_PIXELS);
+  RVK_HIP(hipSetDevice(ctx->device));
+  float* tmp = nullptr;
+  int rc = scratch_on(ctx->device, &tmp);
+  if (rc) return rc;
+  dim3 grid((width + 15) / 16, (height + 15) / 16);
+  // every pixel writes its normalised depth to tmp before the copy back
+  hipLaunchKernelGGL(post_shade_kernel<false>, grid, dim3(256), 0, ctx->stream, width, height, *shade, depth,
+                     min_depth, max_depth, (uint32_t*)rgba, tmp);
+  RVK_HIP(hipGetLastError());
+  RVK_HIP(hipMemcpyAsync(depth, tmp, sizeof(float) * (size_t)width * height, hipMemcpyDeviceToDevice, ctx->stream));
+  return RVK_OK;
 }
 
 int rvk_post_shade_inex(rvk_ctx* ctx, int width, int height, const rvk_shade_params* shade, float* depth,

@@ -467,6 +467,29 @@ __device__ __forceinline__ float trilinear(Cells<T, L, D> vol, const VolGeom& g,
   return lerp(c0, c1, c.az);
 }
 
+// trilinear() with the cell's raw bits kept in registers: consecutive
+// samples of a ray (half a voxel apart) mostly fall in the cell of the one
+// before, and then no load is issued (only the lanes that moved to another
+// cell load, so the L1 sees fewer accesses).  Same operations as trilinear().
+template <typename T, int L> struct CellCache {
+  uint32_t idx = 0xffffffffu;
+  typename CellLoad<T>::Raw raw{};
+  template <bool CLAMP = true, bool D>
+  __device__ __forceinline__ float sample(Cells<T, L, D> vol, const VolGeom& g, float cx, float cy, float cz) {
+    const CellPos c = cell_pos<L, CLAMP>(g, cx, cy, cz);
+    if (c.cell != idx) {
+      raw = fetch_cell<T, L>(vol.d, c.cell);
+      idx = c.cell;
+    }
+    float a[4], d[4];
+    CellLoad<T>::template unpack_ad<D>(raw, a, d);
+    const float c00 = fmaf(c.ax, d[0], a[0]), c10 = fmaf(c.ax, d[1], a[1]);
+    const float c01 = fmaf(c.ax, d[2], a[2]), c11 = fmaf(c.ax, d[3], a[3]);
+    const float c0 = lerp(c00, c10, c.ay), c1 = lerp(c01, c11, c.ay);
+    return lerp(c0, c1, c.az);
+  }
+};
+
 // Trilinear value and its gradient with respect to p (voxel index units) on
 // the same clamped cell, at the cell coordinate c = p + 0.5.  The value is
 // computed with exactly the operations of trilinear(), so branch decisions

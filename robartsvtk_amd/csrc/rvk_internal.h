@@ -56,6 +56,11 @@ struct rvk_tf2d {
   int n = 0;
   Tf2dCell* cells = nullptr;   // n*n
   Tf2dCell* kcells = nullptr;  // keyhole TF or null
+  // The cells lit for the gradient-magnitude path, where the Phong-Lambert
+  // factor is 1: (alpha, sat((ambient + diffuse) r + specular), ... g, ... b),
+  // computed on the device with the caster's own operations.
+  float4* lit = nullptr;
+  float4* klit = nullptr;
   // Opacity bitmaps (bit c = alpha[c] > 0), staged in LDS by the 2D caster
   // when they fit (n <= kTf2dOccMaxN); words = ceil(n*n / 32) each.
   uint32_t* occ = nullptr;

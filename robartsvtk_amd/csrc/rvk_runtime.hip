@@ -795,6 +795,29 @@ static int upload_columns(int n, const float* const* tabs, int count, uint32_t**
   return RVK_OK;
 }
 
+// A cell's contribution in the gradient-magnitude path (Phong-Lambert factor
+// 1), with the operations cast2d_kernel would apply per sample.
+__global__ __launch_bounds__(256) void tf2d_lit_kernel(const Tf2dCell* __restrict__ c, float4* __restrict__ lit,
+                                                       long long nn) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const Tf2dCell e = c[i];
+  const float pl = 1.0f;
+  const float shd = e.ambient + e.diffuse * pl;
+  const float shs = e.specular * powf(pl, e.spec_power);
+  lit[i] = make_float4(e.alpha, satf(shd * e.r + shs), satf(shd * e.g + shs), satf(shd * e.b + shs));
+}
+
+static int make_lit(rvk_ctx* ctx, int n, const Tf2dCell* cells, float4** dst) {
+  const long long nn = (long long)n * n;
+  RVK_HIP(hipMalloc((void**)dst, nn * sizeof(float4)));
+  hipLaunchKernelGGL(tf2d_lit_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->stream, cells, *dst, nn);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) { hipFree(*dst); *dst = nullptr; return hip_fail(e, "tf2d lit cells"); }
+  return RVK_OK;
+}
+
 static int upload_cells(int n, const float* const tables[8], Tf2dCell** dst) {
   for (int i = 0; i < 8; ++i)
     if (!tables[i]) return fail(RVK_ERR_INVALID_ARGUMENT, "2D TF table %d is NULL", i);
@@ -828,6 +851,8 @@ int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const flo
   t->p = *p;
   int rc = upload_cells(n, tables, &t->cells);
   if (rc == RVK_OK && ktables) rc = upload_cells(n, ktables, &t->kcells);
+  if (rc == RVK_OK) rc = make_lit(ctx, n, t->cells, &t->lit);
+  if (rc == RVK_OK && ktables) rc = make_lit(ctx, n, t->kcells, &t->klit);
   if (rc == RVK_OK && n <= kTf2dOccMaxN) {
     t->occ_words = (int)(((size_t)n * n + 31) / 32);
     rc = upload_occupancy(n, tables[0], &t->occ);
@@ -841,6 +866,8 @@ int rvk_tf2d_create(rvk_ctx* ctx, int n, const float* const tables[8], const flo
   if (rc != RVK_OK) {
     if (t->cells) hipFree(t->cells);
     if (t->kcells) hipFree(t->kcells);
+    if (t->lit) hipFree(t->lit);
+    if (t->klit) hipFree(t->klit);
     if (t->occ) hipFree(t->occ);
     if (t->kocc) hipFree(t->kocc);
     if (t->col) hipFree(t->col);
@@ -909,6 +936,8 @@ int rvk_tf2d_destroy(rvk_tf2d* t) {
   hipStreamSynchronize(t->ctx->stream);
   if (t->cells) hipFree(t->cells);
   if (t->kcells) hipFree(t->kcells);
+  if (t->lit) hipFree(t->lit);
+  if (t->klit) hipFree(t->klit);
   if (t->occ) hipFree(t->occ);
   if (t->kocc) hipFree(t->kocc);
   if (t->inex_flags) hipFree(t->inex_flags);

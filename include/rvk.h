@@ -202,6 +202,13 @@ int rvk_volume_destroy(rvk_volume* vol);
 enum { RVK_CELLS_PLAIN = 0, RVK_CELLS_BRICK4 = 1, RVK_CELLS_PAIR8 = 2 };
 int rvk_volume_set_layout(rvk_volume* vol, int layout);
 int rvk_volume_get_layout(const rvk_volume* vol, int* layout);
+/* Re-encode every loaded frame (and gradient-magnitude volume) of vol in
+ * another cell layout on the device; each frame keeps its encoding, renders
+ * stay identical.  Waits for the context stream; graphs captured on the
+ * volume before are not replayed afterwards (the next call launches
+ * directly).  Needs device memory for one native copy of the frames besides
+ * the cells; on failure the volume is left without frames (upload again). */
+int rvk_volume_relayout(rvk_volume* vol, int layout);
 int rvk_volume_dims(const rvk_volume* vol, int32_t dims[3], double spacing[3], int* dtype, int* n_frames);
 
 /* ---- transfer functions -----------------------------------------------------
@@ -313,11 +320,24 @@ int rvk_drr_plan_offsets(const rvk_drr_plan* plan, int64_t* offsets /* n_views +
  * get_schedule reports the render's. */
 int rvk_drr_plan_set_schedule(rvk_drr_plan* plan, int schedule, int waves_per_simd);
 int rvk_drr_plan_get_schedule(const rvk_drr_plan* plan, int* schedule, int* waves_per_simd);
-/* Time every candidate schedule `reps` times on this plan (writes into out)
- * and keep the fastest; *best_ms (nullable) receives its time.  A plan with
- * pose derivatives also tunes the registration step's forward (the march
- * that produces the pose Jacobians) on its own. */
+/* Wave tile of the batched forward kernels: the 64 rays of a wave as an 8x8,
+ * 16x4 or 4x16 pixel block (results do not depend on it). */
+enum { RVK_TILE_8X8 = 0, RVK_TILE_16X4 = 1, RVK_TILE_4X16 = 2 };
+int rvk_drr_plan_set_tile_shape(rvk_drr_plan* plan, int shape);
+int rvk_drr_plan_get_tile_shape(const rvk_drr_plan* plan, int* shape);
+/* Time every candidate -- schedule x waves per SIMD x wave tile -- `reps`
+ * times on this plan (writes into out) and keep the fastest; *best_ms
+ * (nullable) receives its time.  A plan with pose derivatives also tunes the
+ * registration step's forward (the march that produces the pose Jacobians)
+ * on its own. */
 int rvk_drr_plan_autotune(rvk_drr_plan* plan, float* out, int reps, float* best_ms);
+/* rvk_drr_plan_autotune in every cell layout (brick size) the volume fits,
+ * by rvk_volume_relayout of vol -- the plan's own volume, passed again
+ * because it changes for every plan that renders it -- then vol in the
+ * fastest layout and the plan with that layout's best schedule and tile.
+ * *layout (nullable) receives the RVK_CELLS_* chosen. */
+int rvk_drr_plan_autotune_layout(rvk_drr_plan* plan, rvk_volume* vol, float* out, int reps, float* best_ms,
+                                 int* layout);
 /* Render every view into the packed buffer `out` on the context stream. */
 int rvk_drr_plan_render(rvk_drr_plan* plan, float* out);
 /* Capture render(out) (and, with targets, the registration step below) into

@@ -111,6 +111,7 @@ _SIGS = {
     "rvk_volume_destroy": (_I, [_P]),
     "rvk_volume_set_layout": (_I, [_P, _I]),
     "rvk_volume_get_layout": (_I, [_P, C.POINTER(_I)]),
+    "rvk_volume_relayout": (_I, [_P, _I]),
     "rvk_volume_dims": (_I, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(_I), C.POINTER(_I)]),
     "rvk_tf1d_create": (_I, [_P, _I, _P, _P, _P, _P, _P, C.POINTER(Tf1dParams), C.POINTER(_P)]),
     "rvk_tf1d_destroy": (_I, [_P]),
@@ -137,6 +138,9 @@ _SIGS = {
     "rvk_drr_plan_set_schedule": (_I, [_P, _I, _I]),
     "rvk_drr_plan_get_schedule": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "rvk_drr_plan_autotune": (_I, [_P, _P, _I, C.POINTER(C.c_float)]),
+    "rvk_drr_plan_autotune_layout": (_I, [_P, _P, _P, _I, C.POINTER(C.c_float), C.POINTER(_I)]),
+    "rvk_drr_plan_set_tile_shape": (_I, [_P, _I]),
+    "rvk_drr_plan_get_tile_shape": (_I, [_P, C.POINTER(_I)]),
     "rvk_drr_plan_render": (_I, [_P, _P]),
     "rvk_drr_plan_capture": (_I, [_P, _P, _P, _P, _P]),
     "rvk_drr_plan_ncc": (_I, [_P, _P, _P, _P, _P]),
@@ -286,8 +290,12 @@ class Volume:
             check(lib().rvk_volume_set_frame(self.handle, frame, C.c_void_p(data.data_ptr()), 1))
 
     def set_layout(self, layout: int) -> None:
-        """CELLS_PLAIN or CELLS_BRICK4; before the first set_frame."""
+        """CELLS_PLAIN, CELLS_BRICK4 or CELLS_PAIR8; before the first set_frame."""
         check(lib().rvk_volume_set_layout(self.handle, int(layout)))
+
+    def relayout(self, layout: int) -> None:
+        """Re-encode the loaded frames in another cell layout on the device."""
+        check(lib().rvk_volume_relayout(self.handle, int(layout)))
 
     def layout(self) -> int:
         out = C.c_int()
@@ -423,6 +431,7 @@ def count_ray_steps(ctx: Context, vol: Volume, view: View, march: MarchParams) -
 
 
 SCHED_GRID, SCHED_PERSISTENT = 1, 2
+TILE_8X8, TILE_16X4, TILE_4X16 = 0, 1, 2
 CELLS_PLAIN, CELLS_BRICK4, CELLS_PAIR8 = 0, 1, 2
 
 
@@ -461,10 +470,26 @@ class DrrPlan:
         check(lib().rvk_drr_plan_get_schedule(self.handle, C.byref(s), C.byref(w)))
         return s.value, w.value
 
+    def set_tile_shape(self, shape: int):
+        check(lib().rvk_drr_plan_set_tile_shape(self.handle, shape))
+
+    def tile_shape(self) -> int:
+        s = C.c_int()
+        check(lib().rvk_drr_plan_get_tile_shape(self.handle, C.byref(s)))
+        return s.value
+
     def autotune(self, out, reps: int = 5) -> float:
         ms = C.c_float()
         check(lib().rvk_drr_plan_autotune(self.handle, C.c_void_p(dptr(out)), reps, C.byref(ms)))
         return ms.value
+
+    def autotune_layout(self, vol: "Volume", out, reps: int = 5):
+        """autotune in every cell layout of vol (the plan's volume); returns
+        (best ms, CELLS_* chosen)."""
+        ms, lay = C.c_float(), C.c_int()
+        check(lib().rvk_drr_plan_autotune_layout(self.handle, vol.handle, C.c_void_p(dptr(out)), reps,
+                                                 C.byref(ms), C.byref(lay)))
+        return ms.value, lay.value
 
     def render(self, out):
         check(lib().rvk_drr_plan_render(self.handle, C.c_void_p(dptr(out))))

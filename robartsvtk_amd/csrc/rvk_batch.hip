@@ -44,6 +44,7 @@ struct BatchArgs {
   float* jac;             // 6 x total pixels (SoA) d L_p / d pose_i, or NULL
   long long total;        // pixels in the packed images
   unsigned* queues;       // kNumXcd work-queue heads (persistent schedule)
+  int shape;              // wave tile of the forward kernels (RVK_TILE_*)
 };
 
 // Accumulates S = sum grad mu and T = sum (j+t) grad mu over the accumulated
@@ -82,6 +83,23 @@ __device__ __forceinline__ void item_pixel(int tiles16_x, int local, int lane, i
   x = (macro % tiles16_x) * 16 + (q & 1) * 8 + (lane & 7);
   y = (macro / tiles16_x) * 16 + (q >> 1) * 8 + (lane >> 3);
 }
+// The forward kernels' wave tile (the autotuner's choice; the reductions
+// keep the 8x8 mapping above, so their sums do not depend on it): the four
+// items of a 16x16 block as 8x8 quadrants, 16x4 rows or 4x16 columns.
+__device__ __forceinline__ void item_pixel_shaped(int shape, int tiles16_x, int local, int lane, int& x, int& y) {
+  const int macro = local >> 2, q = local & 3;
+  const int bx = (macro % tiles16_x) * 16, by = (macro / tiles16_x) * 16;
+  if (shape == RVK_TILE_16X4) {
+    x = bx + (lane & 15);
+    y = by + q * 4 + (lane >> 4);
+  } else if (shape == RVK_TILE_4X16) {
+    x = bx + q * 4 + (lane & 3);
+    y = by + (lane >> 2);
+  } else {
+    x = bx + (q & 1) * 8 + (lane & 7);
+    y = by + (q >> 1) * 8 + (lane >> 3);
+  }
+}
 
 // One pixel of one view.  With JAC the march also gathers trilinear gradients
 // and the pixel's pose Jacobian dL/dpose_i = A . dstart_i + B . dinc_i is
@@ -91,7 +109,7 @@ __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int l
   const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
   const BatchView& bv = a.views[v];
   int x, y;
-  item_pixel(bv.tiles16_x, item - a.item_begin[v], lane, x, y);
+  item_pixel_shaped(a.shape, bv.tiles16_x, item - a.item_begin[v], lane, x, y);
   if (x >= bv.v.W || y >= bv.v.H) return;
   const Ray r = form_ray(a.g, bv.v, a.step_len, x, y);
   const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
@@ -627,6 +645,8 @@ struct rvk_drr_plan {
   float* d_jac = nullptr;        // 6 x total pixels, allocated with the plan when it has pose derivatives
   int schedule = RVK_SCHED_GRID;
   int waves_per_simd = 4;
+  int shape = RVK_TILE_8X8;       // wave tile of the render
+  int jac_shape = RVK_TILE_8X8;   // and of the registration forward
   // the fused registration forward (pose Jacobians) has its own schedule:
   // its per-wave work differs enough that the best choice can differ
   int jac_schedule = RVK_SCHED_GRID;
@@ -641,6 +661,7 @@ struct rvk_drr_plan {
   float* g_loss = nullptr;
   float* g_grad = nullptr;
   int g_enc = 0;  // cell encoding of the frame when the graph was captured
+  unsigned g_gen = 0;  // and the volume's generation (its cells' addresses and layout)
   bool g_step = false;  // the graph is a registration step with the pose update
   int ncc_mode = RVK_NCC_CENTERED;
   // device-resident poses (rvk_drr_plan_set_poses)
@@ -696,11 +717,13 @@ BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
   a.jac = p->d_jac;
   a.total = p->pix_off[p->n_views];
   a.queues = p->d_queues;
+  a.shape = p->shape;
   return a;
 }
 
 int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s, bool jac = false) {
   BatchArgs a = batch_args(p, out);
+  a.shape = jac ? p->jac_shape : p->shape;
   const bool skip = p->march.skip_rule != 0, pr = p->march.precise != 0;
   return dispatch_frame(p->vol, p->frame, [&](auto tag) {
     using CV = decltype(tag);
@@ -782,6 +805,12 @@ int launch_register(rvk_drr_plan* p, float* out, const float* tgt, float* loss, 
   hipLaunchKernelGGL(pose_reduce_kernel, dim3(p->n_views), dim3(256), 0, s, p->d_item_begin, p->d_pose_part, grad);
   RVK_HIP(hipGetLastError());
   return RVK_OK;
+}
+
+// A captured graph holds the frame's cell pointer, encoding and layout as
+// they were at capture: replayed only while all three are unchanged.
+bool graph_current(const rvk_drr_plan* p) {
+  return p->exec && p->g_enc == p->vol->enc[p->frame] && p->g_gen == p->vol->generation;
 }
 
 void drop_graph(rvk_drr_plan* p) {
@@ -948,8 +977,10 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
   reps = std::max(1, std::min(reps, 100));
   RVK_HIP(hipSetDevice(p->ctx->device));
   drop_graph(p);
+  // schedule x waves per SIMD x wave tile
   const int cand[][2] = {{RVK_SCHED_GRID, 4}, {RVK_SCHED_PERSISTENT, 2}, {RVK_SCHED_PERSISTENT, 4},
                          {RVK_SCHED_PERSISTENT, 8}};
+  const int shapes[] = {RVK_TILE_8X8, RVK_TILE_16X4, RVK_TILE_4X16};
   hipEvent_t e0, e1;
   RVK_HIP(hipEventCreate(&e0));
   RVK_HIP(hipEventCreate(&e1));
@@ -959,24 +990,30 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
   for (int jac = 0; jac < ((p->d_jac && p->march.precise) ? 2 : 1) && !rc; ++jac) {
     int& sched = jac ? p->jac_schedule : p->schedule;
     int& wps = jac ? p->jac_waves_per_simd : p->waves_per_simd;
+    int& shape = jac ? p->jac_shape : p->shape;
     float best = 1e30f;
-    int bs = sched, bw = wps;
-    for (const auto& c : cand) {
-      sched = c[0];
-      wps = c[1];
-      rc = launch_forward(p, out, p->ctx->stream, jac);  // warm-up
+    int bs = sched, bw = wps, bsh = shape;
+    for (const int sh : shapes) {
+      for (const auto& c : cand) {
+        sched = c[0];
+        wps = c[1];
+        shape = sh;
+        rc = launch_forward(p, out, p->ctx->stream, jac);  // warm-up
+        if (rc) break;
+        hipEventRecord(e0, p->ctx->stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = launch_forward(p, out, p->ctx->stream, jac);
+        hipEventRecord(e1, p->ctx->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= reps;
+        if (ms < best) { best = ms; bs = c[0]; bw = c[1]; bsh = sh; }
+      }
       if (rc) break;
-      hipEventRecord(e0, p->ctx->stream);
-      for (int r = 0; r < reps && !rc; ++r) rc = launch_forward(p, out, p->ctx->stream, jac);
-      hipEventRecord(e1, p->ctx->stream);
-      hipEventSynchronize(e1);
-      float ms = 0.0f;
-      hipEventElapsedTime(&ms, e0, e1);
-      ms /= reps;
-      if (ms < best) { best = ms; bs = c[0]; bw = c[1]; }
     }
     sched = bs;
     wps = bw;
+    shape = bsh;
     if (!jac && best_ms) *best_ms = best;
   }
   hipEventDestroy(e0);
@@ -984,10 +1021,70 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
   return rc;
 }
 
+int rvk_drr_plan_autotune_layout(rvk_drr_plan* p, rvk_volume* vol, float* out, int reps, float* best_ms,
+                                 int* layout) {
+  if (!p || !vol || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (vol != p->vol) return fail(RVK_ERR_INVALID_ARGUMENT, "vol is not the volume the plan renders");
+  int rc = RVK_OK;
+  struct Choice {
+    int layout = -1, sched, wps, shape, jsched, jwps, jshape;
+    float ms = 1e30f;
+  } best;
+  for (const int l : {RVK_CELLS_PLAIN, RVK_CELLS_BRICK4, RVK_CELLS_PAIR8}) {
+    if (rvk_volume_relayout(vol, l) != RVK_OK) {
+      // a layout the volume does not fit (its address limits) is skipped;
+      // any other failure leaves the volume without frames and ends the search
+      if (!vol->frames[p->frame]) return RVK_ERR_OUT_OF_MEMORY;
+      continue;
+    }
+    float ms = 0.0f;
+    rc = rvk_drr_plan_autotune(p, out, reps, &ms);
+    if (rc) return rc;
+    if (ms < best.ms) {
+      best.layout = l;
+      best.ms = ms;
+      best.sched = p->schedule;
+      best.wps = p->waves_per_simd;
+      best.shape = p->shape;
+      best.jsched = p->jac_schedule;
+      best.jwps = p->jac_waves_per_simd;
+      best.jshape = p->jac_shape;
+    }
+  }
+  if (best.layout < 0) return fail(RVK_ERR_INVALID_ARGUMENT, "no cell layout fits the volume");
+  rc = rvk_volume_relayout(vol, best.layout);
+  if (rc) return rc;
+  drop_graph(p);
+  p->schedule = best.sched;
+  p->waves_per_simd = best.wps;
+  p->shape = best.shape;
+  p->jac_schedule = best.jsched;
+  p->jac_waves_per_simd = best.jwps;
+  p->jac_shape = best.jshape;
+  if (best_ms) *best_ms = best.ms;
+  if (layout) *layout = best.layout;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_set_tile_shape(rvk_drr_plan* p, int shape) {
+  if (!p) return fail(RVK_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (shape != RVK_TILE_8X8 && shape != RVK_TILE_16X4 && shape != RVK_TILE_4X16)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown tile shape %d", shape);
+  if (shape != p->shape || shape != p->jac_shape) drop_graph(p);
+  p->shape = p->jac_shape = shape;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_get_tile_shape(const rvk_drr_plan* p, int* shape) {
+  if (!p || !shape) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *shape = p->shape;
+  return RVK_OK;
+}
+
 int rvk_drr_plan_render(rvk_drr_plan* p, float* out) {
   if (!p || !out) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && !p->g_step && p->g_out == out && !p->g_tgt && p->g_enc == p->vol->enc[p->frame]) {
+  if (graph_current(p) && !p->g_step && p->g_out == out && !p->g_tgt) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }
@@ -1018,6 +1115,7 @@ int rvk_drr_plan_capture(rvk_drr_plan* p, float* out, const float* tgt, float* l
   p->g_loss = loss;
   p->g_grad = grad;
   p->g_enc = p->vol->enc[p->frame];
+  p->g_gen = p->vol->generation;
   return RVK_OK;
 }
 
@@ -1025,8 +1123,8 @@ int rvk_drr_plan_register(rvk_drr_plan* p, float* out, const float* tgt, float* 
   if (!p || !out || !tgt || !loss || !grad) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!p->march.precise) return fail(RVK_ERR_UNSUPPORTED, "the adjoint differentiates the precise march");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && !p->g_step && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad &&
-      p->g_enc == p->vol->enc[p->frame]) {
+  if (graph_current(p) && !p->g_step && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss &&
+      p->g_grad == grad) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }
@@ -1179,8 +1277,8 @@ int rvk_drr_plan_step(rvk_drr_plan* p, float* out, const float* tgt, float* loss
   if (!p || !out || !tgt || !loss || !grad) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!p->d_poses) return fail(RVK_ERR_NOT_READY, "the plan has no device poses (rvk_drr_plan_set_poses)");
   RVK_HIP(hipSetDevice(p->ctx->device));
-  if (p->exec && p->g_step && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss && p->g_grad == grad &&
-      p->g_enc == p->vol->enc[p->frame]) {
+  if (graph_current(p) && p->g_step && p->g_out == out && p->g_tgt == tgt && p->g_loss == loss &&
+      p->g_grad == grad) {
     RVK_HIP(hipGraphLaunch(p->exec, p->ctx->stream));
     return RVK_OK;
   }
@@ -1207,6 +1305,7 @@ int rvk_drr_plan_capture_step(rvk_drr_plan* p, float* out, const float* tgt, flo
   p->g_loss = loss;
   p->g_grad = grad;
   p->g_enc = p->vol->enc[p->frame];
+  p->g_gen = p->vol->generation;
   p->g_step = true;
   return RVK_OK;
 }

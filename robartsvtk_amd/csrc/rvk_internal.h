@@ -36,6 +36,9 @@ struct rvk_volume {
   // march's empty-space skipping; nrb = ceil((dims + 2) / kRangeEdge).
   float2* ranges[RVK_MAX_FRAMES] = {};
   int nrb[3] = {0, 0, 0};
+  // bumped whenever the frames' cells move (rvk_volume_relayout): graphs
+  // captured before hold stale cell pointers and are not replayed
+  unsigned generation = 0;
 };
 
 struct rvk_tf1d {

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "rvk_internal.h"
@@ -424,6 +425,26 @@ static int default_layout(const int32_t dims[3], int dtype) {
 using namespace rvk;
 
 // ---------------------------------------------------------------------------
+// Cells -> x-fastest native voxels: voxel (i, j, k) is the first slot of
+// cell (i+1, j+1, k+1) in every layout and encoding (the base of its x-pair,
+// stored exactly; binary16 bases hold integers in [-2048, 2048]).
+template <typename CV, typename VT>
+__global__ __launch_bounds__(256) void extract_voxels_kernel(const void* cell_data, VolGeom g, VT* __restrict__ out,
+                                                             long long total) {
+  const CV cells = cells_of<CV>(cell_data);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int i = (int)(e % g.n[0]);
+  const int j = (int)((e / g.n[0]) % g.n[1]);
+  const int k = (int)(e / ((long long)g.n[0] * g.n[1]));
+  using T = typename CV::value_type;
+  constexpr int L = CV::kLog2;
+  const T raw = cells.d[(size_t)CellOrder<L>::index(g, i + 1, j + 1, k + 1) * CellStride<L>::k];
+  if constexpr (std::is_same<T, Half16>::value || std::is_same<T, HalfPair>::value) out[e] = (VT)(int)(float)raw;
+  else out[e] = (VT)raw;
+}
+
+// ---------------------------------------------------------------------------
 // Streaming-read probe: every 16-byte word of a buffer read once, coalesced,
 // with a grid that fills the device.  The loads feed an xor that is stored
 // only on a match no real buffer produces, so they are not dropped.
@@ -626,6 +647,135 @@ int rvk_volume_get_layout(const rvk_volume* v, int* layout) {
   return RVK_OK;
 }
 
+// Cells of one frame (in the volume's layout and the frame's encoding) from
+// x-fastest native voxels on the device, and (with_ranges) its range bricks.
+static int write_frame_cells(rvk_volume* v, int frame, const void* dsrc, bool with_ranges) {
+  const int enc = v->enc[frame];
+  // dispatched on the source voxel type: binary16 cells are written by the
+  // difference relayout of their 16-bit source
+  const bool half = enc == kEncHalf || enc == kEncHalfPair;
+  return dispatch_cells<false>(v->dtype, v->layout, half ? kEncDiff : enc, [&](auto tag) {
+    using CV = decltype(tag);
+    using VT = typename CV::value_type;
+    int r = RVK_OK;
+    if constexpr (sizeof(VT) == 2 && CV::kDiff) {
+      if (enc == kEncHalfPair) r = launch_relayout<VT, CV::kLog2, true, 2>(v, dsrc, v->frames[frame]);
+      else if (enc == kEncHalf) r = launch_relayout<VT, CV::kLog2, true, 1>(v, dsrc, v->frames[frame]);
+      else r = launch_relayout<VT, CV::kLog2, true>(v, dsrc, v->frames[frame]);
+    } else {
+      r = launch_relayout<VT, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);
+    }
+    if (r == RVK_OK && with_ranges) {
+      const int nb = v->nrb[0] * v->nrb[1] * v->nrb[2];
+      if (!v->ranges[frame]) RVK_HIP(hipMalloc((void**)&v->ranges[frame], sizeof(float2) * (size_t)nb));
+      hipLaunchKernelGGL((brick_range_kernel<VT>), dim3((nb + 255) / 256), dim3(256), 0, v->ctx->stream,
+                         (const VT*)dsrc, v->dims[0], v->dims[1], v->dims[2], v->nrb[0], v->nrb[1], v->nrb[2],
+                         v->ranges[frame]);
+      RVK_HIP(hipGetLastError());
+    }
+    return r;
+  });
+}
+
+int rvk_volume_relayout(rvk_volume* v, int layout) {
+  if (!v) return fail(RVK_ERR_INVALID_ARGUMENT, "volume is NULL");
+  if (layout != RVK_CELLS_PLAIN && layout != RVK_CELLS_BRICK4 && layout != RVK_CELLS_PAIR8)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown cell layout %d", layout);
+  const int target = layout == RVK_CELLS_BRICK4 ? kCellLayoutBrick4
+                     : layout == RVK_CELLS_PAIR8 ? kCellLayoutPair8
+                                                 : kCellLayoutPlain;
+  if (target == v->layout) return RVK_OK;
+  bool any = false;
+  for (int f = 0; f < RVK_MAX_FRAMES; ++f) any = any || v->frames[f];
+  if (!any) return rvk_volume_set_layout(v, layout);
+  {
+    rvk_volume probe = *v;  // the new layout must fit its addressing limits
+    probe.layout = target;
+    const int rc = size_cells(&probe);
+    if (rc) return rc;
+  }
+  RVK_HIP(hipSetDevice(v->ctx->device));
+  RVK_HIP(hipStreamSynchronize(v->ctx->stream));
+  const size_t n = (size_t)v->dims[0] * v->dims[1] * v->dims[2];
+  const size_t es = dtype_size(v->dtype);
+  const VolGeom g = vol_geom(v);
+  const int blocks = (int)((n + 255) / 256);
+  std::vector<void*> lin(RVK_MAX_FRAMES, nullptr), glin(RVK_MAX_FRAMES, nullptr);
+  auto release = [&]() {
+    for (void* q : lin) if (q) hipFree(q);
+    for (void* q : glin) if (q) hipFree(q);
+  };
+  int rc = RVK_OK;
+  // every frame (and gradient-magnitude volume) back to native voxels
+  for (int f = 0; f < RVK_MAX_FRAMES && !rc; ++f) {
+    if (!v->frames[f]) continue;
+    if (hipMalloc(&lin[f], n * es) != hipSuccess) { rc = fail(RVK_ERR_OUT_OF_MEMORY, "relayout staging"); break; }
+    rc = dispatch_frame(v, f, [&](auto tag) {
+      using CV = decltype(tag);
+      auto go = [&](auto vt) {
+        using VT = decltype(vt);
+        hipLaunchKernelGGL((extract_voxels_kernel<CV, VT>), dim3(blocks), dim3(256), 0, v->ctx->stream,
+                           (const void*)v->frames[f], g, (VT*)lin[f], (long long)n);
+        return (int)RVK_OK;
+      };
+      switch (v->dtype) {
+        case RVK_U8: return go(uint8_t{});
+        case RVK_I8: return go(int8_t{});
+        case RVK_U16: return go(uint16_t{});
+        case RVK_I16: return go(int16_t{});
+        case RVK_U32: return go(uint32_t{});
+        case RVK_I32: return go(int32_t{});
+        default: return go(float{});
+      }
+    });
+    if (!rc && v->gradmag[f]) {
+      if (hipMalloc(&glin[f], n) != hipSuccess) { rc = fail(RVK_ERR_OUT_OF_MEMORY, "relayout staging"); break; }
+      auto gm = [&](auto tagL) {
+        constexpr int L = decltype(tagL)::value;
+        hipLaunchKernelGGL((extract_voxels_kernel<Cells<uint8_t, L>, uint8_t>), dim3(blocks), dim3(256), 0,
+                           v->ctx->stream, (const void*)v->gradmag[f], g, (uint8_t*)glin[f], (long long)n);
+      };
+      if (v->layout == kCellLayoutBrick4) gm(std::integral_constant<int, kCellLayoutBrick4>{});
+      else if (v->layout == kCellLayoutPair8) gm(std::integral_constant<int, kCellLayoutPair8>{});
+      else gm(std::integral_constant<int, kCellLayoutPlain>{});
+    }
+  }
+  if (!rc) {
+    const hipError_t e = hipStreamSynchronize(v->ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "relayout extraction");
+  }
+  if (rc) { release(); return rc; }
+  // the new layout: cells rewritten from the native voxels, same encodings
+  for (int f = 0; f < RVK_MAX_FRAMES; ++f) {
+    if (v->frames[f]) { hipFree(v->frames[f]); v->frames[f] = nullptr; }
+    if (v->gradmag[f]) { hipFree(v->gradmag[f]); v->gradmag[f] = nullptr; }
+  }
+  v->layout = target;
+  size_cells(v);
+  for (int f = 0; f < RVK_MAX_FRAMES && !rc; ++f) {
+    if (!lin[f]) continue;
+    if (hipMalloc(&v->frames[f], v->n_cells * 8 * es) != hipSuccess) {
+      rc = fail(RVK_ERR_OUT_OF_MEMORY, "relayout cells");
+      break;
+    }
+    rc = write_frame_cells(v, f, lin[f], false);
+    if (!rc && glin[f]) {
+      if (hipMalloc((void**)&v->gradmag[f], v->n_cells * 8) != hipSuccess) {
+        rc = fail(RVK_ERR_OUT_OF_MEMORY, "relayout cells");
+        break;
+      }
+      if (v->layout == kCellLayoutBrick4) rc = launch_relayout<uint8_t, kCellLayoutBrick4>(v, glin[f], v->gradmag[f]);
+      else if (v->layout == kCellLayoutPair8) rc = launch_relayout<uint8_t, kCellLayoutPair8>(v, glin[f], v->gradmag[f]);
+      else rc = launch_relayout<uint8_t, kCellLayoutPlain>(v, glin[f], v->gradmag[f]);
+    }
+  }
+  const hipError_t e = hipStreamSynchronize(v->ctx->stream);
+  release();
+  ++v->generation;
+  if (!rc && e != hipSuccess) rc = hip_fail(e, "relayout");
+  return rc;
+}
+
 int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_device) {
   if (!v || !src) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
   if (frame < 0 || frame >= v->n_frames)
@@ -660,30 +810,7 @@ int rvk_volume_set_frame(rvk_volume* v, int frame, const void* src, int src_on_d
   }
   if (rc == RVK_OK) {
     v->enc[frame] = (uint8_t)enc;
-    // dispatched on the source voxel type: binary16 cells are written by the
-    // difference relayout of their 16-bit source
-    const bool half = enc == kEncHalf || enc == kEncHalfPair;
-    rc = dispatch_cells<false>(v->dtype, v->layout, half ? kEncDiff : enc, [&](auto tag) {
-      using CV = decltype(tag);
-      using VT = typename CV::value_type;
-      int r = RVK_OK;
-      if constexpr (sizeof(VT) == 2 && CV::kDiff) {
-        if (enc == kEncHalfPair) r = launch_relayout<VT, CV::kLog2, true, 2>(v, dsrc, v->frames[frame]);
-        else if (enc == kEncHalf) r = launch_relayout<VT, CV::kLog2, true, 1>(v, dsrc, v->frames[frame]);
-        else r = launch_relayout<VT, CV::kLog2, true>(v, dsrc, v->frames[frame]);
-      } else {
-        r = launch_relayout<VT, CV::kLog2, CV::kDiff>(v, dsrc, v->frames[frame]);
-      }
-      if (r == RVK_OK) {
-        const int nb = v->nrb[0] * v->nrb[1] * v->nrb[2];
-        if (!v->ranges[frame]) RVK_HIP(hipMalloc((void**)&v->ranges[frame], sizeof(float2) * (size_t)nb));
-        hipLaunchKernelGGL((brick_range_kernel<VT>), dim3((nb + 255) / 256), dim3(256), 0, v->ctx->stream,
-                           (const VT*)dsrc, v->dims[0], v->dims[1], v->dims[2], v->nrb[0], v->nrb[1], v->nrb[2],
-                           v->ranges[frame]);
-        RVK_HIP(hipGetLastError());
-      }
-      return r;
-    });
+    rc = write_frame_cells(v, frame, dsrc, true);
   }
   if (staging) {
     hipError_t e = hipStreamSynchronize(v->ctx->stream);

@@ -150,6 +150,54 @@ def test_autotune_and_graph_replays(gpu_ctx):
     plan.close()
 
 
+def test_tile_shapes_and_layout_autotune(gpu_ctx):
+    """Every wave tile renders the same images; the layout autotuner
+    re-encodes the plan's volume, after which a graph captured before is not
+    replayed (its cell pointers are stale) and renders stay identical."""
+    torch = _torch()
+    s = configs.scene_c4(scale=0.125, n_poses=12)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    mp, dp = configs.march_params(s.march), configs.drr_params(s.drr)
+    plan = abi.DrrPlan(gpu_ctx, vol, s.views, mp, dp)
+    ref = torch.zeros(plan.total, dtype=torch.float32, device="cuda")
+    plan.render(ref)
+    torch.cuda.synchronize()
+    out = torch.empty_like(ref)
+    for shape in (abi.TILE_16X4, abi.TILE_4X16, abi.TILE_8X8):
+        for sched in (abi.SCHED_GRID, abi.SCHED_PERSISTENT):
+            plan.set_tile_shape(shape)
+            plan.set_schedule(sched, 4)
+            assert plan.tile_shape() == shape
+            out.fill_(float("nan"))
+            plan.render(out)
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (shape, sched)
+    with pytest.raises(abi.RvkError):
+        plan.set_tile_shape(3)
+    plan.capture(out)
+    other = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    with pytest.raises(abi.RvkError):
+        plan.autotune_layout(other, out, reps=1)   # not the plan's volume
+    other.close()
+    ms, layout = plan.autotune_layout(vol, out, reps=2)
+    assert ms > 0 and vol.layout() == layout
+    assert plan.tile_shape() in (abi.TILE_8X8, abi.TILE_16X4, abi.TILE_4X16)
+    for rep in range(2):
+        out.fill_(-1.0)
+        plan.render(out)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+    for layout in (abi.CELLS_PAIR8, abi.CELLS_BRICK4, abi.CELLS_PLAIN):
+        plan.capture(out)
+        vol.relayout(layout)
+        out.fill_(-1.0)
+        plan.render(out)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), layout
+    plan.close()
+    vol.close()
+
+
 # ---------------------------------------------------------------------------
 def _registration_scene(n_views=3, scale=0.125):
     s = configs.scene_c5(scale=scale, n_views=n_views)

@@ -526,6 +526,53 @@ def test_cell_layouts_render_identically(gpu_ctx):
     assert outs[abi.CELLS_PLAIN][2][..., 3].max() > 100
 
 
+def test_relayout_renders_identically(gpu_ctx):
+    """rvk_volume_relayout re-encodes loaded frames (u8, int16 in x-pair
+    binary16, u16 with its gradient-magnitude volume, float32 differences)
+    in every other layout; renders stay bit-identical to a fresh upload."""
+    s2 = configs.scene_c2(scale=0.25)
+    s1 = configs.scene_c1(scale=0.25)
+    s3 = _scene_c3_small()
+    rng = np.random.default_rng(11)
+    f32 = rng.normal(0.0, 300.0, s2.volume.shape).astype(np.float32)
+
+    def render(v2, v1, v3, vf):
+        return (H.gpu_drr(gpu_ctx, v2, s2.views[0], s2.march, s2.drr),
+                H.gpu_1d(gpu_ctx, v1, s1.tf1d, s1.views[0], dict(s1.march, term_T=1.0 / 64)),
+                H.gpu_2d(gpu_ctx, v3, s3.tf2d, s3.views[0], s3.march, use_gradmag=True),
+                H.gpu_drr(gpu_ctx, vf, s2.views[0], s2.march, s2.drr))
+
+    def upload(layout):
+        v2 = abi.Volume.from_array(gpu_ctx, s2.volume, s2.spacing, layout=layout)
+        v1 = abi.Volume.from_array(gpu_ctx, s1.volume, s1.spacing, layout=layout)
+        v3 = abi.Volume.from_array(gpu_ctx, s3.volume, s3.spacing, layout=layout)
+        v3.compute_gradmag(0, s3.tf2d["gradmag_max"])
+        vf = abi.Volume.from_array(gpu_ctx, f32, s2.spacing, layout=layout)
+        return v2, v1, v3, vf
+
+    vols = upload(abi.CELLS_PLAIN)
+    ref = render(*vols)
+    for layout in (abi.CELLS_PAIR8, abi.CELLS_BRICK4, abi.CELLS_PLAIN, abi.CELLS_PAIR8):
+        for v in vols:
+            v.relayout(layout)
+            assert v.layout() == layout
+        for a, b in zip(ref, render(*vols)):
+            assert np.array_equal(a, b), layout
+    for v in vols:
+        v.close()
+    assert ref[2][..., 3].max() > 100
+    # a layout beyond the volume's addressing limits is refused and the
+    # volume keeps its frames and layout
+    thin = abi.Volume(gpu_ctx, (2, 4096, 4096), (1.0, 1.0, 1.0), np.uint8)
+    thin.set_frame(0, np.zeros((4096, 4096, 2), np.uint8))
+    with pytest.raises(abi.RvkError):
+        thin.relayout(abi.CELLS_PLAIN)
+    assert thin.layout() == abi.CELLS_BRICK4
+    thin.relayout(abi.CELLS_PAIR8)
+    assert thin.layout() == abi.CELLS_PAIR8
+    thin.close()
+
+
 def test_cell_layout_rules(gpu_ctx):
     small = abi.Volume(gpu_ctx, (64, 64, 64), (1.0, 1.0, 1.0), np.int16)
     assert small.layout() == abi.CELLS_PLAIN          # 35 MB of cells

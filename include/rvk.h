@@ -370,12 +370,12 @@ int rvk_drr_plan_set_ncc(rvk_drr_plan* plan, int mode);
  * rvk_pose_matrix; RVK_ERR_NOT_READY when the plan has no dview) and/or
  * d loss / d volume (voxel values, x fastest; float32 volumes with the plain
  * march only; samples in a view's keyhole window are excluded, as in the
- * forward).  Rays are re-marched with the forward's control flow; nothing per
- * sample is stored.
+ * forward).  For the pose gradient rays are re-marched with the forward's
+ * control flow; nothing per sample is stored.
  * The volume gradient is a voxel-driven gather (each voxel re-marches only
  * the samples its trilinear support touches) with no atomics; its first call
- * allocates a per-pixel ray record buffer inside the plan.  Gradients are
- * those of the precise march. */
+ * allocates a per-pixel ray record buffer (64 bytes per pixel) inside the
+ * plan.  Gradients are those of the precise march. */
 int rvk_drr_plan_backward(rvk_drr_plan* plan, const float* images, const float* dloss_dimage, float* grad_pose,
                           float* grad_volume);
 /* The registration step, fused: render every view into out, NCC against

@@ -398,20 +398,22 @@ __global__ __launch_bounds__(256) void jac_grad_kernel(JacGradArgs a) {
 // the sample.  Each voxel projects its support (p in (i-0.5, i+1.5)^3) onto
 // every detector, intersects the covered rays with the support, and re-marches
 // only those samples; one thread owns one voxel, so there are no atomics.
-struct RayRec {
-  float p0x, p0y, p0z, ix, iy, iz;  // start in cell coordinates (p0 + 0.5) and step
-  int n;
-  float cs, ci;   // alpha = cs * value + ci
-  float coef;     // d loss / d L_p * ds
-  float kw0, kw1; // keyhole window: samples t in [kw0, kw1) are never accumulated
+struct alignas(16) RayRec {
+  float p0x, p0y, p0z;  // start in cell coordinates (p0 + 0.5)
+  float coef;           // d loss / d L_p * ds * slope
+  float ix, iy, iz;     // step
+  int n;                // samples
+  float cs, ci;         // alpha = cs * value + ci
+  float kw0, kw1;       // keyhole window: samples t in [kw0, kw1) are never accumulated
 };
+static_assert(sizeof(RayRec) == 48, "RayRec is three 16-byte vectors");
 
 struct VolGradArgs {
   BatchArgs b;
   const float* img;
   const float* dldi;
-  RayRec* rays;   // packed like the images
-  float* grad;    // dims x-fastest
+  RayRec* rays;      // packed like the images
+  float* grad;       // dims x-fastest
 };
 
 __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
@@ -429,8 +431,7 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
   const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
   const float I = a.img[pi];
   const float dLdI = a.dldi[pi];
-  const float ds = sqrtf(r.inc.x * r.inc.x * g.sp[0] * g.sp[0] + r.inc.y * r.inc.y * g.sp[1] * g.sp[1] +
-                         r.inc.z * r.inc.z * g.sp[2] * g.sp[2]);
+  const float ds = ray_length(g, r.inc);
   RayRec rr;
   rr.p0x = (r.start.x + j * r.inc.x) + 0.5f;
   rr.p0y = (r.start.y + j * r.inc.y) + 0.5f;
@@ -504,7 +505,10 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
     for (int v = 0; v < a.b.n_views; ++v) {
       const BatchView& bv = a.b.views[v];
       const int W = bv.v.W, H = bv.v.H;
-      // pixel box of the support's 8 corners (in sample coordinates p = c - 0.5)
+      // pixel box of the support's 8 corners (in sample coordinates p = c -
+      // 0.5).  Pixel x's ray projects to x exactly, so the box keeps a
+      // margin of a whole pixel beyond the rays that can meet the support,
+      // and v_rcp_f32 (1 ulp) serves for the perspective divide
       float xmin = 1e30f, xmax = -1e30f, ymin = 1e30f, ymax = -1e30f;
       bool behind = false;
       for (int c = 0; c < 8; ++c) {
@@ -515,7 +519,8 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
         const float hy = M[4] * px + M[5] * py + M[6] * pz + M[7];
         const float hw = M[12] * px + M[13] * py + M[14] * pz + M[15];
         if (!(hw > 0.0f)) { behind = true; break; }
-        const float vx = hx / hw, vy = hy / hw;
+        const float rw = __builtin_amdgcn_rcpf(hw);
+        const float vx = hx * rw, vy = hy * rw;
         const float xp = bv.v.flipped ? vx * (float)W : (1.0f - vx) * (float)W;
         const float yp = vy * (float)H;
         xmin = fminf(xmin, xp); xmax = fmaxf(xmax, xp);
@@ -526,33 +531,40 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
         x0 = max(0, (int)floorf(xmin) - 1); x1 = min(W - 1, (int)ceilf(xmax) + 1);
         y0 = max(0, (int)floorf(ymin) - 1); y1 = min(H - 1, (int)ceilf(ymax) + 1);
       }
+      const float4* rec = reinterpret_cast<const float4*>(a.rays + bv.pix_off);
       for (int py = y0; py <= y1; ++py)
         for (int px = x0; px <= x1; ++px) {
-          const RayRec rr = a.rays[bv.pix_off + (long long)py * W + px];
-          if (rr.n <= 0 || rr.coef == 0.0f) continue;
-          // sample indices m with c0 + m inc inside the open support box
-          float tlo = 0.0f, thi = (float)(rr.n - 1);
-          const float p0[3] = {rr.p0x, rr.p0y, rr.p0z}, in[3] = {rr.ix, rr.iy, rr.iz};
+          const long long q = 3 * ((long long)py * W + px);
+          const float4 A = rec[q], B = rec[q + 1], K = rec[q + 2];
+          const int n = __float_as_int(B.w);
+          if (n <= 0 || A.w == 0.0f) continue;
+          // sample indices m with c0 + m inc inside the open support box: the
+          // entry and exit times from v_rcp_f32 reciprocals differ from the
+          // quotients by a few ulp, far less than the one-sample margin each
+          // side, and every sample in the range is tested by its weight
+          float tlo = 0.0f, thi = (float)(n - 1);
+          const float p0[3] = {A.x, A.y, A.z}, in[3] = {B.x, B.y, B.z};
 #pragma unroll
           for (int ax = 0; ax < 3; ++ax) {
             if (in[ax] != 0.0f) {
-              const float ta = (clo[ax] - p0[ax]) / in[ax], tb = (chi[ax] - p0[ax]) / in[ax];
+              const float rin = __builtin_amdgcn_rcpf(in[ax]);
+              const float ta = (clo[ax] - p0[ax]) * rin, tb = (chi[ax] - p0[ax]) * rin;
               tlo = fmaxf(tlo, fminf(ta, tb));
               thi = fminf(thi, fmaxf(ta, tb));
             } else if (p0[ax] <= clo[ax] || p0[ax] >= chi[ax]) {
               thi = -1.0f;
             }
           }
-          const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(rr.n - 1, (int)ceilf(thi) + 1);
+          const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(n - 1, (int)ceilf(thi) + 1);
           for (int m = m0; m <= m1; ++m) {
             const float t = (float)m;
             // samples in the keyhole window are not accumulated by the forward
-            if (t >= rr.kw0 && t < rr.kw1) continue;
-            const float qx = fmaf(t, rr.ix, rr.p0x), qy = fmaf(t, rr.iy, rr.p0y), qz = fmaf(t, rr.iz, rr.p0z);
+            if (t >= K.z && t < K.w) continue;
+            const float qx = fmaf(t, B.x, A.x), qy = fmaf(t, B.y, A.y), qz = fmaf(t, B.z, A.z);
             const float w = axis_weight(qx, i, g.n[0]) * axis_weight(qy, jj, g.n[1]) * axis_weight(qz, k, g.n[2]);
             if (w <= 0.0f) continue;
-            const float alpha = fmaf(rr.cs, trilinear(vol, g, qx, qy, qz), rr.ci);
-            if (alpha < 0.0f) acc += rr.coef * w;
+            const float alpha = fmaf(K.x, trilinear(vol, g, qx, qy, qz), K.y);
+            if (alpha < 0.0f) acc += A.w * w;
           }
         }
     }
@@ -1176,7 +1188,8 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
       return fail(RVK_ERR_UNSUPPORTED, "the volume gradient is defined for float32 volumes");
     if (p->march.skip_rule || p->drr.ref_termination)
       return fail(RVK_ERR_UNSUPPORTED, "the volume gradient needs the plain march (no skip rule, no termination)");
-    if (!p->d_rays) RVK_HIP(hipMalloc((void**)&p->d_rays, sizeof(RayRec) * (size_t)p->pix_off[p->n_views]));
+    const long long total = p->pix_off[p->n_views];
+    if (!p->d_rays) RVK_HIP(hipMalloc((void**)&p->d_rays, sizeof(RayRec) * (size_t)total));
     VolGradArgs va;
     va.b = batch_args(p, nullptr);
     va.img = images;
@@ -1185,15 +1198,10 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     va.grad = grad_vol;
     hipLaunchKernelGGL(ray_record_kernel, dim3(blocks), dim3(256), 0, s, va);
     const long long nvox = (long long)p->vol->dims[0] * p->vol->dims[1] * p->vol->dims[2];
-    if (p->vol->layout == kCellLayoutBrick4)
-      hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutBrick4>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                         s, va);
-    else if (p->vol->layout == kCellLayoutPair8)
-      hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPair8>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0,
-                         s, va);
-    else
-      hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPlain>, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s,
-                         va);
+    const dim3 vgrid((unsigned)((nvox + 255) / 256));
+    if (p->vol->layout == kCellLayoutBrick4) hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutBrick4>, vgrid, dim3(256), 0, s, va);
+    else if (p->vol->layout == kCellLayoutPair8) hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPair8>, vgrid, dim3(256), 0, s, va);
+    else hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPlain>, vgrid, dim3(256), 0, s, va);
     RVK_HIP(hipGetLastError());
   }
   return RVK_OK;

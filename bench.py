@@ -256,7 +256,10 @@ def bench_batched(scene, reps: int = 5, volume_gradient: bool = False, loop: boo
            "fwd_speedup": round(t_plain / t_batch, 3),
            "batched_fwd_G_samples_per_s": round(n_samples / (t_batch / 1e3) / 1e9, 2),
            "schedule": {"kind": "persistent" if sched[0] == abi.SCHED_PERSISTENT else "grid",
-                        "waves_per_simd": sched[1], "autotune_ms": round(tuned_ms, 3)},
+                        "waves_per_simd": sched[1],
+                        "tile": {abi.TILE_8X8: "8x8", abi.TILE_16X4: "16x4", abi.TILE_4X16: "4x16"}[plan.tile_shape()],
+                        "work_order": "regions" if plan.work_order() == abi.ORDER_REGIONS else "views",
+                        "autotune_ms": round(tuned_ms, 3)},
            "plain_fwd_fd_ms": round(t_plain_fd, 3), "batched_fwd_bwd_ms": round(t_reg, 3),
            "plain_fd_poses_per_s": round(n / (t_plain_fd / 1e3), 2),
            "batched_fwd_bwd_poses_per_s": round(n / (t_reg / 1e3), 2),

@@ -325,7 +325,15 @@ int rvk_drr_plan_get_schedule(const rvk_drr_plan* plan, int* schedule, int* wave
 enum { RVK_TILE_8X8 = 0, RVK_TILE_16X4 = 1, RVK_TILE_4X16 = 2 };
 int rvk_drr_plan_set_tile_shape(rvk_drr_plan* plan, int shape);
 int rvk_drr_plan_get_tile_shape(const rvk_drr_plan* plan, int* shape);
-/* Time every candidate -- schedule x waves per SIMD x wave tile -- `reps`
+/* Order in which the batched forward takes the views' tiles (results do not
+ * depend on it): view after view, or grouped by the volume region each
+ * tile's central ray crosses, all views' tiles of a region together, so one
+ * XCD's L2 serves that region's cells to every view (recomputed when the
+ * views are updated from the host). */
+enum { RVK_ORDER_VIEWS = 0, RVK_ORDER_REGIONS = 1 };
+int rvk_drr_plan_set_work_order(rvk_drr_plan* plan, int order);
+int rvk_drr_plan_get_work_order(const rvk_drr_plan* plan, int* order);
+/* Time every candidate -- schedule x waves per SIMD x wave tile x work order -- `reps`
  * times on this plan (writes into out) and keep the fastest; *best_ms
  * (nullable) receives its time.  A plan with pose derivatives also tunes the
  * registration step's forward (the march that produces the pose Jacobians)

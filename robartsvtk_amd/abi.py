@@ -139,6 +139,8 @@ _SIGS = {
     "rvk_drr_plan_get_schedule": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "rvk_drr_plan_autotune": (_I, [_P, _P, _I, C.POINTER(C.c_float)]),
     "rvk_drr_plan_autotune_layout": (_I, [_P, _P, _P, _I, C.POINTER(C.c_float), C.POINTER(_I)]),
+    "rvk_drr_plan_set_work_order": (_I, [_P, _I]),
+    "rvk_drr_plan_get_work_order": (_I, [_P, C.POINTER(_I)]),
     "rvk_drr_plan_set_tile_shape": (_I, [_P, _I]),
     "rvk_drr_plan_get_tile_shape": (_I, [_P, C.POINTER(_I)]),
     "rvk_drr_plan_render": (_I, [_P, _P]),
@@ -432,6 +434,7 @@ def count_ray_steps(ctx: Context, vol: Volume, view: View, march: MarchParams) -
 
 SCHED_GRID, SCHED_PERSISTENT = 1, 2
 TILE_8X8, TILE_16X4, TILE_4X16 = 0, 1, 2
+ORDER_VIEWS, ORDER_REGIONS = 0, 1
 CELLS_PLAIN, CELLS_BRICK4, CELLS_PAIR8 = 0, 1, 2
 
 
@@ -469,6 +472,14 @@ class DrrPlan:
         s, w = C.c_int(), C.c_int()
         check(lib().rvk_drr_plan_get_schedule(self.handle, C.byref(s), C.byref(w)))
         return s.value, w.value
+
+    def set_work_order(self, order: int):
+        check(lib().rvk_drr_plan_set_work_order(self.handle, order))
+
+    def work_order(self) -> int:
+        o = C.c_int()
+        check(lib().rvk_drr_plan_get_work_order(self.handle, C.byref(o)))
+        return o.value
 
     def set_tile_shape(self, shape: int):
         check(lib().rvk_drr_plan_set_tile_shape(self.handle, shape))

@@ -45,6 +45,7 @@ struct BatchArgs {
   long long total;        // pixels in the packed images
   unsigned* queues;       // kNumXcd work-queue heads (persistent schedule)
   int shape;              // wave tile of the forward kernels (RVK_TILE_*)
+  const int2* work;       // forward work order: (view, tile of the view) per work item
 };
 
 // Accumulates S = sum grad mu and T = sum (j+t) grad mu over the accumulated
@@ -106,10 +107,13 @@ __device__ __forceinline__ void item_pixel_shaped(int shape, int tiles16_x, int 
 // written next to the image (the registration step's single march).
 template <typename CV, bool SKIP, bool PRECISE, bool JAC, bool TERM>
 __device__ __forceinline__ void forward_item(const BatchArgs& a, int item, int lane) {
-  const int v = __builtin_amdgcn_readfirstlane(find_view(a.item_begin, a.n_views, item));
+  // the forward writes only its own pixels, so it may take the tiles in any
+  // order (the reductions keep the view-major one)
+  const int2 vt = a.work[item];
+  const int v = __builtin_amdgcn_readfirstlane(vt.x);
   const BatchView& bv = a.views[v];
   int x, y;
-  item_pixel_shaped(a.shape, bv.tiles16_x, item - a.item_begin[v], lane, x, y);
+  item_pixel_shaped(a.shape, bv.tiles16_x, __builtin_amdgcn_readfirstlane(vt.y), lane, x, y);
   if (x >= bv.v.W || y >= bv.v.H) return;
   const Ray r = form_ray(a.g, bv.v, a.step_len, x, y);
   const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
@@ -659,6 +663,11 @@ struct rvk_drr_plan {
   int waves_per_simd = 4;
   int shape = RVK_TILE_8X8;       // wave tile of the render
   int jac_shape = RVK_TILE_8X8;   // and of the registration forward
+  // forward work orders: view-major, or grouped by the volume region the
+  // tile's central ray crosses (all views' tiles of one region together, so
+  // an XCD's L2 serves one region's cells to every view)
+  int2* d_work[2] = {nullptr, nullptr};
+  int order = 0, jac_order = 0;
   // the fused registration forward (pose Jacobians) has its own schedule:
   // its per-wave work differs enough that the best choice can differ
   int jac_schedule = RVK_SCHED_GRID;
@@ -707,6 +716,83 @@ int fill_views(rvk_drr_plan* p, const rvk_view* views, const float* dview, std::
   return RVK_OK;
 }
 
+// Work orders of the forward.  Order 0 is view-major.  Order 1 sorts the
+// 16x16 tiles (4 work items each, kept together) by the volume region their
+// central ray crosses: the midpoint of the ray's chord through the volume
+// box, on a 32^3 grid of regions in Morton order (c4 forward: 39.2 ms
+// view-major, 35.3 ms on a 16^3 grid, 35.0 ms on 32^3, 36.3 ms on 8^3);
+// tiles whose ray misses the volume go last.  Ties keep view-major order.
+static uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t k = 0;
+  for (int b = 0; b < 5; ++b) k |= (((x >> b) & 1u) << (3 * b)) | (((y >> b) & 1u) << (3 * b + 1)) | (((z >> b) & 1u) << (3 * b + 2));
+  return k;
+}
+
+static void work_orders(const rvk_drr_plan* p, const rvk_view* views, std::vector<int2>& by_view,
+                        std::vector<int2>& by_region) {
+  by_view.resize(p->n_items);
+  for (int v = 0; v < p->n_views; ++v)
+    for (int i = p->item_begin[v]; i < p->item_begin[v + 1]; ++i) by_view[i] = make_int2(v, i - p->item_begin[v]);
+  const int* dims = p->vol->dims;
+  const double cell = std::max(1.0, std::max(dims[0], std::max(dims[1], dims[2])) / 32.0);
+  struct Tile { uint32_t key; int v, macro; };
+  std::vector<Tile> tiles;
+  tiles.reserve(p->n_items / 4);
+  for (int v = 0; v < p->n_views; ++v) {
+    const rvk_view& vw = views[v];
+    const float* m = vw.view_to_voxels;
+    const int tx = (vw.width + 15) / 16, n_macro = (p->item_begin[v + 1] - p->item_begin[v]) / 4;
+    for (int mc = 0; mc < n_macro; ++mc) {
+      const double x = std::min(vw.width - 1, (mc % tx) * 16 + 8), y = std::min(vw.height - 1, (mc / tx) * 16 + 8);
+      const double vx = vw.flipped ? x / vw.width : 1.0 - x / vw.width, vy = y / vw.height;
+      double pt[2][3];
+      bool ok = true;
+      for (int d = 0; d < 2; ++d) {
+        const double w = vx * m[12] + vy * m[13] + d * m[14] + m[15];
+        ok = ok && w != 0.0;
+        for (int a = 0; a < 3; ++a) pt[d][a] = (vx * m[4 * a] + vy * m[4 * a + 1] + d * m[4 * a + 2] + m[4 * a + 3]) / (ok ? w : 1.0);
+      }
+      // the line's chord through the box [0, n - 1]^3 (slab clipping)
+      double t0 = -1e300, t1 = 1e300;
+      for (int a = 0; a < 3 && ok; ++a) {
+        const double dir = pt[1][a] - pt[0][a];
+        if (dir == 0.0) {
+          ok = pt[0][a] >= 0.0 && pt[0][a] <= dims[a] - 1;
+        } else {
+          double ta = (0.0 - pt[0][a]) / dir, tb = (dims[a] - 1 - pt[0][a]) / dir;
+          if (ta > tb) std::swap(ta, tb);
+          t0 = std::max(t0, ta);
+          t1 = std::min(t1, tb);
+        }
+      }
+      uint32_t key = 0xffffffffu;
+      if (ok && t0 <= t1) {
+        const double tm = 0.5 * (t0 + t1);
+        uint32_t g[3];
+        for (int a = 0; a < 3; ++a)
+          g[a] = (uint32_t)std::min(31.0, std::max(0.0, std::floor((pt[0][a] + tm * (pt[1][a] - pt[0][a])) / cell)));
+        key = morton3(g[0], g[1], g[2]);
+      }
+      tiles.push_back(Tile{key, v, mc});
+    }
+  }
+  std::stable_sort(tiles.begin(), tiles.end(), [](const Tile& a, const Tile& b) { return a.key < b.key; });
+  by_region.resize(p->n_items);
+  size_t i = 0;
+  for (const Tile& tl : tiles)
+    for (int q = 0; q < 4; ++q) by_region[i++] = make_int2(tl.v, 4 * tl.macro + q);
+}
+
+static int upload_work_orders(rvk_drr_plan* p, const rvk_view* views) {
+  std::vector<int2> o[2];
+  work_orders(p, views, o[0], o[1]);
+  for (int k = 0; k < 2; ++k) {
+    if (!p->d_work[k]) RVK_HIP(hipMalloc((void**)&p->d_work[k], sizeof(int2) * (size_t)p->n_items));
+    RVK_HIP(hipMemcpy(p->d_work[k], o[k].data(), sizeof(int2) * (size_t)p->n_items, hipMemcpyHostToDevice));
+  }
+  return RVK_OK;
+}
+
 BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
   BatchArgs a;
   a.g = vol_geom(p->vol);
@@ -730,12 +816,14 @@ BatchArgs batch_args(const rvk_drr_plan* p, float* out) {
   a.total = p->pix_off[p->n_views];
   a.queues = p->d_queues;
   a.shape = p->shape;
+  a.work = p->d_work[p->order];
   return a;
 }
 
 int launch_forward(rvk_drr_plan* p, float* out, hipStream_t s, bool jac = false) {
   BatchArgs a = batch_args(p, out);
   a.shape = jac ? p->jac_shape : p->shape;
+  a.work = p->d_work[jac ? p->jac_order : p->order];
   const bool skip = p->march.skip_rule != 0, pr = p->march.precise != 0;
   return dispatch_frame(p->vol, p->frame, [&](auto tag) {
     using CV = decltype(tag);
@@ -919,6 +1007,11 @@ int rvk_drr_plan_create(rvk_ctx* ctx, const rvk_volume* vol, int frame, int n_vi
     rvk_drr_plan_destroy(p);
     return hip_fail(e, "plan allocation");
   }
+  rc = upload_work_orders(p, views);
+  if (rc) {
+    rvk_drr_plan_destroy(p);
+    return rc;
+  }
   *out = p;
   return RVK_OK;
 }
@@ -931,6 +1024,8 @@ int rvk_drr_plan_destroy(rvk_drr_plan* p) {
   if (p->cap_stream) hipStreamDestroy(p->cap_stream);
   hipFree(p->d_views);
   hipFree(p->d_item_begin);
+  hipFree(p->d_work[0]);
+  hipFree(p->d_work[1]);
   hipFree(p->d_queues);
   hipFree(p->d_part);
   hipFree(p->d_stats);
@@ -955,7 +1050,8 @@ int rvk_drr_plan_update_views(rvk_drr_plan* p, const rvk_view* views, const floa
   RVK_HIP(hipMemcpyAsync(p->d_views, hv.data(), sizeof(BatchView) * p->n_views, hipMemcpyHostToDevice,
                          p->ctx->stream));
   RVK_HIP(hipStreamSynchronize(p->ctx->stream));
-  return RVK_OK;
+  // new views, new regions (a captured graph reads the arrays, not their contents)
+  return upload_work_orders(p, views);
 }
 
 int rvk_drr_plan_offsets(const rvk_drr_plan* p, int64_t* offsets) {
@@ -989,7 +1085,7 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
   reps = std::max(1, std::min(reps, 100));
   RVK_HIP(hipSetDevice(p->ctx->device));
   drop_graph(p);
-  // schedule x waves per SIMD x wave tile
+  // schedule x waves per SIMD x wave tile x work order
   const int cand[][2] = {{RVK_SCHED_GRID, 4}, {RVK_SCHED_PERSISTENT, 2}, {RVK_SCHED_PERSISTENT, 4},
                          {RVK_SCHED_PERSISTENT, 8}};
   const int shapes[] = {RVK_TILE_8X8, RVK_TILE_16X4, RVK_TILE_4X16};
@@ -1003,13 +1099,16 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
     int& sched = jac ? p->jac_schedule : p->schedule;
     int& wps = jac ? p->jac_waves_per_simd : p->waves_per_simd;
     int& shape = jac ? p->jac_shape : p->shape;
+    int& order = jac ? p->jac_order : p->order;
     float best = 1e30f;
-    int bs = sched, bw = wps, bsh = shape;
-    for (const int sh : shapes) {
+    int bs = sched, bw = wps, bsh = shape, bo = order;
+    for (int oi = 0; oi < 2 * 3 && !rc; ++oi) {
+      const int o = oi / 3, sh = shapes[oi % 3];
       for (const auto& c : cand) {
         sched = c[0];
         wps = c[1];
         shape = sh;
+        order = o;
         rc = launch_forward(p, out, p->ctx->stream, jac);  // warm-up
         if (rc) break;
         hipEventRecord(e0, p->ctx->stream);
@@ -1019,13 +1118,13 @@ int rvk_drr_plan_autotune(rvk_drr_plan* p, float* out, int reps, float* best_ms)
         float ms = 0.0f;
         hipEventElapsedTime(&ms, e0, e1);
         ms /= reps;
-        if (ms < best) { best = ms; bs = c[0]; bw = c[1]; bsh = sh; }
+        if (ms < best) { best = ms; bs = c[0]; bw = c[1]; bsh = sh; bo = o; }
       }
-      if (rc) break;
     }
     sched = bs;
     wps = bw;
     shape = bsh;
+    order = bo;
     if (!jac && best_ms) *best_ms = best;
   }
   hipEventDestroy(e0);
@@ -1039,7 +1138,7 @@ int rvk_drr_plan_autotune_layout(rvk_drr_plan* p, rvk_volume* vol, float* out, i
   if (vol != p->vol) return fail(RVK_ERR_INVALID_ARGUMENT, "vol is not the volume the plan renders");
   int rc = RVK_OK;
   struct Choice {
-    int layout = -1, sched, wps, shape, jsched, jwps, jshape;
+    int layout = -1, sched, wps, shape, order, jsched, jwps, jshape, jorder;
     float ms = 1e30f;
   } best;
   for (const int l : {RVK_CELLS_PLAIN, RVK_CELLS_BRICK4, RVK_CELLS_PAIR8}) {
@@ -1058,9 +1157,11 @@ int rvk_drr_plan_autotune_layout(rvk_drr_plan* p, rvk_volume* vol, float* out, i
       best.sched = p->schedule;
       best.wps = p->waves_per_simd;
       best.shape = p->shape;
+      best.order = p->order;
       best.jsched = p->jac_schedule;
       best.jwps = p->jac_waves_per_simd;
       best.jshape = p->jac_shape;
+      best.jorder = p->jac_order;
     }
   }
   if (best.layout < 0) return fail(RVK_ERR_INVALID_ARGUMENT, "no cell layout fits the volume");
@@ -1070,11 +1171,28 @@ int rvk_drr_plan_autotune_layout(rvk_drr_plan* p, rvk_volume* vol, float* out, i
   p->schedule = best.sched;
   p->waves_per_simd = best.wps;
   p->shape = best.shape;
+  p->order = best.order;
   p->jac_schedule = best.jsched;
   p->jac_waves_per_simd = best.jwps;
   p->jac_shape = best.jshape;
+  p->jac_order = best.jorder;
   if (best_ms) *best_ms = best.ms;
   if (layout) *layout = best.layout;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_set_work_order(rvk_drr_plan* p, int order) {
+  if (!p) return fail(RVK_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (order != RVK_ORDER_VIEWS && order != RVK_ORDER_REGIONS)
+    return fail(RVK_ERR_INVALID_ARGUMENT, "unknown work order %d", order);
+  if (order != p->order || order != p->jac_order) drop_graph(p);
+  p->order = p->jac_order = order;
+  return RVK_OK;
+}
+
+int rvk_drr_plan_get_work_order(const rvk_drr_plan* p, int* order) {
+  if (!p || !order) return fail(RVK_ERR_INVALID_ARGUMENT, "NULL argument");
+  *order = p->order;
   return RVK_OK;
 }
 

@@ -151,7 +151,7 @@ def test_autotune_and_graph_replays(gpu_ctx):
 
 
 def test_tile_shapes_and_layout_autotune(gpu_ctx):
-    """Every wave tile renders the same images; the layout autotuner
+    """Every wave tile and work order renders the same images; the layout autotuner
     re-encodes the plan's volume, after which a graph captured before is not
     replayed (its cell pointers are stale) and renders stay identical."""
     torch = _torch()
@@ -163,17 +163,21 @@ def test_tile_shapes_and_layout_autotune(gpu_ctx):
     plan.render(ref)
     torch.cuda.synchronize()
     out = torch.empty_like(ref)
-    for shape in (abi.TILE_16X4, abi.TILE_4X16, abi.TILE_8X8):
-        for sched in (abi.SCHED_GRID, abi.SCHED_PERSISTENT):
-            plan.set_tile_shape(shape)
-            plan.set_schedule(sched, 4)
-            assert plan.tile_shape() == shape
-            out.fill_(float("nan"))
-            plan.render(out)
-            torch.cuda.synchronize()
-            assert torch.equal(out, ref), (shape, sched)
+    for order in (abi.ORDER_REGIONS, abi.ORDER_VIEWS):
+        for shape in (abi.TILE_16X4, abi.TILE_4X16, abi.TILE_8X8):
+            for sched in (abi.SCHED_GRID, abi.SCHED_PERSISTENT):
+                plan.set_work_order(order)
+                plan.set_tile_shape(shape)
+                plan.set_schedule(sched, 4)
+                assert plan.tile_shape() == shape and plan.work_order() == order
+                out.fill_(float("nan"))
+                plan.render(out)
+                torch.cuda.synchronize()
+                assert torch.equal(out, ref), (order, shape, sched)
     with pytest.raises(abi.RvkError):
         plan.set_tile_shape(3)
+    with pytest.raises(abi.RvkError):
+        plan.set_work_order(2)
     plan.capture(out)
     other = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
     with pytest.raises(abi.RvkError):

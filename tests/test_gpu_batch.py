@@ -66,10 +66,12 @@ def _with_keyhole(view, n):
     return view
 
 
-def test_batch_views_with_keyhole_planes(gpu_ctx):
+@pytest.mark.parametrize("order", [abi.ORDER_VIEWS, abi.ORDER_REGIONS])
+def test_batch_views_with_keyhole_planes(gpu_ctx, order):
     """A plan with keyhole planes on some views renders every view as the
     single-view caster does (which specialises views without planes), also
-    after update_views adds or removes the planes of a captured plan."""
+    after update_views adds or removes the planes of a captured plan (and,
+    in the region work order, recomputes the order)."""
     torch = _torch()
     s = configs.scene_c4(scale=0.125, n_poses=6)
     n = s.dims[0]
@@ -86,6 +88,7 @@ def test_batch_views_with_keyhole_planes(gpu_ctx):
             assert torch.equal(plan.split(out)[v], single), v
 
     plan = abi.DrrPlan(gpu_ctx, vol, plain, mp, dp)
+    plan.set_work_order(order)
     out = torch.zeros(plan.total, dtype=torch.float32, device="cuda")
     plan.capture(out)
     plan.render(out)

@@ -173,3 +173,34 @@ def test_device_registration_loop_recovers_poses(gpu_ctx, shared):
     assert e_rot.max() < 1.0 and e_tr.max() < 2.0
     plan.close()
     vol.close()
+
+
+def test_device_steps_identical_across_relayout_and_work_order(gpu_ctx):
+    """Three replayed registration steps give the same images, losses,
+    gradients and poses when, between the capture and the replays, the
+    volume is re-encoded in another cell layout (the stale graph is not
+    replayed: the steps are launched directly) or the forward takes the
+    tiles in the region order."""
+    import torch
+    s, tg = _registration_scene(n_views=3)
+    runs = []
+    for case in ("plain", "relayout", "regions"):
+        vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+        plan = _plan_with_derivatives(gpu_ctx, s, vol)
+        plan.set_poses(s.extra["cameras"], s.extra["center"], s.extra["poses"], _adam(False))
+        tgt, out, loss, grad = _tensors(torch, plan, tg)
+        if case == "regions":
+            plan.set_work_order(abi.ORDER_REGIONS)
+        plan.capture_step(out, tgt, loss, grad)
+        if case == "relayout":
+            vol.relayout(abi.CELLS_BRICK4 if vol.layout() != abi.CELLS_BRICK4 else abi.CELLS_PLAIN)
+        for _ in range(3):
+            plan.step(out, tgt, loss, grad)
+        torch.cuda.synchronize()
+        runs.append((out.cpu(), loss.cpu(), grad.cpu(), plan.poses()))
+        plan.close()
+        vol.close()
+    for other in runs[1:]:
+        for a, b in zip(runs[0][:3], other[:3]):
+            assert torch.equal(a, b)
+        np.testing.assert_array_equal(runs[0][3], other[3])

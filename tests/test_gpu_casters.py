@@ -499,6 +499,24 @@ def test_plain_march_sample_count(gpu_ctx, which):
     assert n64 > 0 and abs(n_gpu - n64) <= 1e-3 * n64
 
 
+def test_drr_sample_count_against_fp64_oracle(gpu_ctx):
+    """The DRR benchmark's sample count, rvk_count_ray_steps (geometry only:
+    the sum over pixels of floor(numSteps - offset)), against the FP64
+    oracle's plain-march count on the c2 geometry at half size, and the
+    samples the skip-rule march evaluates (FP32 oracle) below it."""
+    s = configs.scene_c2(scale=0.5)
+    vh = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    view = s.views[0]
+    n_gpu = abi.count_ray_steps(gpu_ctx, vh, view, configs.march_params(s.march))
+    vh.close()
+    vd = configs.view_dict(view)
+    n64 = orc.count_samples(s.volume, s.spacing, vd, dict(s.march, skip_rule=0))
+    n_eval = orc.count_drr_evaluated(s.volume, s.spacing, vd, s.march, s.drr)
+    print(f"c2/2: plain-march samples GPU {n_gpu}, FP64 oracle {n64}, skip-rule evaluated {n_eval}")
+    assert n64 > 0 and abs(n_gpu - n64) <= 1e-5 * n64
+    assert 0 < n_eval <= n64
+
+
 # ---------------------------------------------------------------------------
 # cell order: plain x-fastest and 4x4x4 bricks give identical renders
 def test_cell_layouts_render_identically(gpu_ctx):

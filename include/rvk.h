@@ -206,8 +206,9 @@ int rvk_volume_get_layout(const rvk_volume* vol, int* layout);
  * another cell layout on the device; each frame keeps its encoding, renders
  * stay identical.  Waits for the context stream; graphs captured on the
  * volume before are not replayed afterwards (the next call launches
- * directly).  Needs device memory for one native copy of the frames besides
- * the cells; on failure the volume is left without frames (upload again). */
+ * directly).  Needs device memory for the new cells and one native copy of
+ * the frames beside the old cells; when it cannot allocate them it fails
+ * with RVK_ERR_OUT_OF_MEMORY and leaves the volume unchanged. */
 int rvk_volume_relayout(rvk_volume* vol, int layout);
 int rvk_volume_dims(const rvk_volume* vol, int32_t dims[3], double spacing[3], int* dtype, int* n_frames);
 

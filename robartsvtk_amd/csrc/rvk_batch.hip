@@ -1142,12 +1142,10 @@ int rvk_drr_plan_autotune_layout(rvk_drr_plan* p, rvk_volume* vol, float* out, i
     float ms = 1e30f;
   } best;
   for (const int l : {RVK_CELLS_PLAIN, RVK_CELLS_BRICK4, RVK_CELLS_PAIR8}) {
-    if (rvk_volume_relayout(vol, l) != RVK_OK) {
-      // a layout the volume does not fit (its address limits) is skipped;
-      // any other failure leaves the volume without frames and ends the search
-      if (!vol->frames[p->frame]) return RVK_ERR_OUT_OF_MEMORY;
-      continue;
-    }
+    // a layout the volume does not fit (its address limits, or the device
+    // memory for a second copy of its cells) is skipped; the volume keeps
+    // its cells then
+    if (rvk_volume_relayout(vol, l) != RVK_OK) continue;
     float ms = 0.0f;
     rc = rvk_drr_plan_autotune(p, out, reps, &ms);
     if (rc) return rc;

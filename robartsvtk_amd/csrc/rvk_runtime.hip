@@ -745,25 +745,36 @@ int rvk_volume_relayout(rvk_volume* v, int layout) {
     if (e != hipSuccess) rc = hip_fail(e, "relayout extraction");
   }
   if (rc) { release(); return rc; }
-  // the new layout: cells rewritten from the native voxels, same encodings
-  for (int f = 0; f < RVK_MAX_FRAMES; ++f) {
-    if (v->frames[f]) { hipFree(v->frames[f]); v->frames[f] = nullptr; }
-    if (v->gradmag[f]) { hipFree(v->gradmag[f]); v->gradmag[f] = nullptr; }
+  // the new cells are allocated before the old ones are released, so that a
+  // failed allocation leaves the volume as it was
+  rvk_volume probe = *v;
+  probe.layout = target;
+  size_cells(&probe);
+  std::vector<void*> nf(RVK_MAX_FRAMES, nullptr);
+  std::vector<uint8_t*> ng(RVK_MAX_FRAMES, nullptr);
+  for (int f = 0; f < RVK_MAX_FRAMES && !rc; ++f) {
+    if (lin[f] && hipMalloc(&nf[f], probe.n_cells * 8 * es) != hipSuccess) rc = RVK_ERR_OUT_OF_MEMORY;
+    if (!rc && glin[f] && hipMalloc((void**)&ng[f], probe.n_cells * 8) != hipSuccess) rc = RVK_ERR_OUT_OF_MEMORY;
   }
+  if (rc) {
+    for (void* q : nf) if (q) hipFree(q);
+    for (uint8_t* q : ng) if (q) hipFree(q);
+    release();
+    return fail(RVK_ERR_OUT_OF_MEMORY, "relayout: no device memory for the new cells (volume unchanged)");
+  }
+  // the new layout: cells rewritten from the native voxels, same encodings
+  std::vector<void*> of(v->frames, v->frames + RVK_MAX_FRAMES);
+  std::vector<uint8_t*> og(v->gradmag, v->gradmag + RVK_MAX_FRAMES);
   v->layout = target;
   size_cells(v);
+  for (int f = 0; f < RVK_MAX_FRAMES; ++f) {
+    v->frames[f] = nf[f];
+    v->gradmag[f] = ng[f];
+  }
   for (int f = 0; f < RVK_MAX_FRAMES && !rc; ++f) {
     if (!lin[f]) continue;
-    if (hipMalloc(&v->frames[f], v->n_cells * 8 * es) != hipSuccess) {
-      rc = fail(RVK_ERR_OUT_OF_MEMORY, "relayout cells");
-      break;
-    }
     rc = write_frame_cells(v, f, lin[f], false);
     if (!rc && glin[f]) {
-      if (hipMalloc((void**)&v->gradmag[f], v->n_cells * 8) != hipSuccess) {
-        rc = fail(RVK_ERR_OUT_OF_MEMORY, "relayout cells");
-        break;
-      }
       if (v->layout == kCellLayoutBrick4) rc = launch_relayout<uint8_t, kCellLayoutBrick4>(v, glin[f], v->gradmag[f]);
       else if (v->layout == kCellLayoutPair8) rc = launch_relayout<uint8_t, kCellLayoutPair8>(v, glin[f], v->gradmag[f]);
       else rc = launch_relayout<uint8_t, kCellLayoutPlain>(v, glin[f], v->gradmag[f]);
@@ -771,6 +782,8 @@ int rvk_volume_relayout(rvk_volume* v, int layout) {
   }
   const hipError_t e = hipStreamSynchronize(v->ctx->stream);
   release();
+  for (void* q : of) if (q) hipFree(q);
+  for (uint8_t* q : og) if (q) hipFree(q);
   ++v->generation;
   if (!rc && e != hipSuccess) rc = hip_fail(e, "relayout");
   return rc;

@@ -272,8 +272,8 @@ def bench_batched(scene, reps: int = 5, volume_gradient: bool = False, loop: boo
         t_vg = _time(lambda: plan.backward(out, dldi, grad_volume=gvol), max(1, reps // 2))
         assert bool(torch.isfinite(gvol).all())
         res["volume_gradient_ms"] = round(t_vg, 3)
-        res["volume_gradient_note"] = (f"rvk_drr_plan_backward(grad_volume) alone: ray records + voxel-driven gather "
-                                       f"over {scene.volume.size} voxels x {n} views")
+        res["volume_gradient_note"] = (f"rvk_drr_plan_backward(grad_volume) alone: ray records + brick-driven "
+                                       f"fixed-point scatter over {scene.volume.size} voxels x {n} views")
     if loop:
         opt = abi.PoseOptimizer(kind=abi.OPT_ADAM, shared_pose=0, lr_rotation=0.005, lr_translation=0.5, beta1=0.9,
                                 beta2=0.999, eps=1e-8)

@@ -381,10 +381,15 @@ int rvk_drr_plan_set_ncc(rvk_drr_plan* plan, int mode);
  * march only; samples in a view's keyhole window are excluded, as in the
  * forward).  For the pose gradient rays are re-marched with the forward's
  * control flow; nothing per sample is stored.
- * The volume gradient is a voxel-driven gather (each voxel re-marches only
- * the samples its trilinear support touches) with no atomics; its first call
- * allocates a per-pixel ray record buffer (64 bytes per pixel) inside the
- * plan.  Gradients are those of the precise march. */
+ * The volume gradient is brick-driven: each 8^3-voxel brick re-marches the
+ * stretch of every ray that meets the cells its voxels weigh in and adds each
+ * accumulated sample's trilinear weights times the ray's coefficient into
+ * 64-bit fixed-point accumulators (a multiple of 2^-k, 2^k set per call so
+ * that the largest |coefficient| is below 2^30), so the result is the same on
+ * every call.  A non-finite d loss / d image on a ray with samples makes every
+ * voxel whose neighbourhood can contribute NaN.  Its first call allocates a
+ * per-pixel ray record buffer (48 bytes per pixel) and 4 bytes per 8x8 pixel
+ * tile inside the plan.  Gradients are those of the precise march. */
 int rvk_drr_plan_backward(rvk_drr_plan* plan, const float* images, const float* dloss_dimage, float* grad_pose,
                           float* grad_volume);
 /* The registration step, fused: render every view into out, NCC against

@@ -12,7 +12,7 @@ the differentiable inputs.
     loss.backward()                            # poses.grad from the HIP adjoint
 
 The forward is rvk_drr_plan_render and the backward rvk_drr_plan_backward
-(pose adjoint and, when the volume requires grad, the voxel-driven volume
+(pose adjoint and, when the volume requires grad, the brick-driven volume
 adjoint).  The plan holds one set of views and one volume frame; every forward
 records which poses and volume it rendered, and its backward loads them back
 into the plan when another forward has replaced them since (several renders in

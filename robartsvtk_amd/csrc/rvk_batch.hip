@@ -397,11 +397,17 @@ __global__ __launch_bounds__(256) void jac_grad_kernel(JacGradArgs a) {
 
 // ---------------------------------------------------------------------------
 // Volume adjoint (f32 mu volumes, plain march): per pixel a ray record, then
-// a voxel-driven gather.  d loss / d mu_ijk = sum_views sum_pixels coef_p *
-// ds * sum_m [mu(p_m) > 0] w_ijk(p_m), w the trilinear weight of voxel ijk at
-// the sample.  Each voxel projects its support (p in (i-0.5, i+1.5)^3) onto
-// every detector, intersects the covered rays with the support, and re-marches
-// only those samples; one thread owns one voxel, so there are no atomics.
+// a brick-driven scatter.  d loss / d mu_ijk = sum_views sum_pixels coef_p *
+// sum_m [alpha(p_m) < 0] w_ijk(p_m), w the trilinear weight of voxel ijk at
+// the sample and coef_p = d loss / d L_p * ds * slope.  One workgroup owns an
+// 8^3-voxel brick: it marches every ray that can meet the cells its voxels
+// weigh in over that stretch only, and each accumulated sample adds its 8
+// corner contributions to the brick's accumulators in LDS.  The accumulators
+// are 64-bit fixed point (contributions rounded to multiples of 2^-k, with
+// 2^k set per call from the largest |coef| so that one contribution stays
+// below 2^30): integer sums do not depend on the order of the adds, so the
+// result is the same whichever wave adds first, and LDS integer atomics are
+// far cheaper than float ones.
 struct alignas(16) RayRec {
   float p0x, p0y, p0z;  // start in cell coordinates (p0 + 0.5)
   float coef;           // d loss / d L_p * ds * slope
@@ -417,8 +423,12 @@ struct VolGradArgs {
   const float* img;
   const float* dldi;
   RayRec* rays;      // packed like the images
+  float* coef_max;   // per 8x8 tile: the largest |coef| of its rays (+inf if one is not finite)
+  int* scale_log2;   // k of the fixed-point accumulation, or kVgNonFinite
   float* grad;       // dims x-fastest
 };
+
+constexpr int kVgNonFinite = 1 << 30;
 
 __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
   const int item = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -428,152 +438,242 @@ __global__ __launch_bounds__(256) void ray_record_kernel(VolGradArgs a) {
   const BatchView& bv = a.b.views[v];
   int x, y;
   item_pixel(bv.tiles16_x, item - a.b.item_begin[v], lane, x, y);
-  if (x >= bv.v.W || y >= bv.v.H) return;
-  const VolGeom& g = a.b.g;
-  const Ray r = form_ray(g, bv.v, a.b.step_len, x, y);
-  const float j = ray_offset(a.b.m.jitter, x, y);
-  const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
-  const float I = a.img[pi];
-  const float dLdI = a.dldi[pi];
-  const float ds = ray_length(g, r.inc);
-  RayRec rr;
-  rr.p0x = (r.start.x + j * r.inc.x) + 0.5f;
-  rr.p0y = (r.start.y + j * r.inc.y) + 0.5f;
-  rr.p0z = (r.start.z + j * r.inc.z) + 0.5f;
-  rr.ix = r.inc.x; rr.iy = r.inc.y; rr.iz = r.inc.z;
-  rr.n = step_count(r.num_steps - j);
-  // the rays the precise forward march takes no samples of (drr_march)
-  if (!samples_in_cells(g, V3<float>{rr.p0x, rr.p0y, rr.p0z}, r.inc, rr.n)) rr.n = 0;
-  // the keyhole window exactly as drr_march forms it (t = kst - k)
-  const int es = rr.n - (int)ceilf(r.exclude_start), ee = rr.n - (int)floorf(r.exclude_end);
-  rr.kw0 = (float)(rr.n - es);
-  rr.kw1 = (float)(rr.n - ee);
-  rr.ci = a.b.m.intercept * -ds;
-  rr.cs = a.b.m.slope * -ds;
-  // d loss / d mu at a sample = (d loss / d L) * ds * slope, with dL/dI = -I for exp(-L)
-  const float dLdL = (a.b.m.output == RVK_DRR_F32_EXP_NEG_L) ? -dLdI * I : dLdI;
-  rr.coef = dLdL * ds * a.b.m.slope;
-  a.rays[pi] = rr;
+  float amax = 0.0f;
+  if (x < bv.v.W && y < bv.v.H) {
+    const VolGeom& g = a.b.g;
+    const Ray r = form_ray(g, bv.v, a.b.step_len, x, y);
+    const float j = ray_offset(a.b.m.jitter, x, y);
+    const long long pi = bv.pix_off + (long long)y * bv.v.W + x;
+    const float I = a.img[pi];
+    const float dLdI = a.dldi[pi];
+    const float ds = ray_length(g, r.inc);
+    RayRec rr;
+    rr.p0x = (r.start.x + j * r.inc.x) + 0.5f;
+    rr.p0y = (r.start.y + j * r.inc.y) + 0.5f;
+    rr.p0z = (r.start.z + j * r.inc.z) + 0.5f;
+    rr.ix = r.inc.x; rr.iy = r.inc.y; rr.iz = r.inc.z;
+    rr.n = step_count(r.num_steps - j);
+    // the rays the precise forward march takes no samples of (drr_march)
+    if (!samples_in_cells(g, V3<float>{rr.p0x, rr.p0y, rr.p0z}, r.inc, rr.n)) rr.n = 0;
+    // the keyhole window exactly as drr_march forms it (t = kst - k)
+    const int es = rr.n - (int)ceilf(r.exclude_start), ee = rr.n - (int)floorf(r.exclude_end);
+    rr.kw0 = (float)(rr.n - es);
+    rr.kw1 = (float)(rr.n - ee);
+    rr.ci = a.b.m.intercept * -ds;
+    rr.cs = a.b.m.slope * -ds;
+    // d loss / d mu at a sample = (d loss / d L) * ds * slope, with dL/dI = -I for exp(-L)
+    const float dLdL = (a.b.m.output == RVK_DRR_F32_EXP_NEG_L) ? -dLdI * I : dLdI;
+    rr.coef = dLdL * ds * a.b.m.slope;
+    a.rays[pi] = rr;
+    // rays without samples add nothing; a non-finite coefficient poisons the call
+    if (rr.n > 0) amax = isfinite(rr.coef) ? fabsf(rr.coef) : __builtin_huge_valf();
+  }
+  for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+  if (lane == 0) a.coef_max[item] = amax;
 }
 
-__device__ __forceinline__ float hat(float t) { return fmaxf(0.0f, 1.0f - fabsf(t)); }
+// k = 30 - e for the largest |coef| = f 2^e (f in [0.5, 1)): every scaled
+// contribution |coef w 2^k| (w <= 1) is below 2^30 and rounds into an int32.
+__global__ __launch_bounds__(1024) void vg_scale_kernel(const float* __restrict__ coef_max, int n, int* scale_log2) {
+  __shared__ float s_m[16];
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < n; i += 1024) m = fmaxf(m, coef_max[i]);
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) m = fmaxf(m, s_m[w]);
+    int e = 0;
+    if (m > 0.0f) frexpf(m, &e);
+    scale_log2[0] = isfinite(m) ? 30 - e : kVgNonFinite;
+  }
+}
 
-// Trilinear weight, along one axis, of voxel i (of n) at the cell coordinate
-// c: the hat centred on c = i + 1, except that the edge cells hold a clamped
-// voxel twice (cell 0 = voxels (0, 0), cell n = (n - 1, n - 1)), so voxel 0
-// weighs 1 for c < 1 and voxel n - 1 weighs 1 for c > n.
-__device__ __forceinline__ float axis_weight(float c, int i, int n) {
-  const float ci = (float)(i + 1);
-  if ((i == 0 && c < ci) || (i == n - 1 && c > ci)) return 1.0f;
-  return hat(c - ci);
+// The LDS cube holds voxels [8b - 2, 8b + 9] per axis.  A sample's lower
+// corner index, clamped into [0, 10], is the true one whenever either corner
+// is a brick voxel; otherwise both land in the halo, which is dropped, except
+// that at the volume's faces the clamped edge voxels are folded in (voxel -1
+// holds voxel 0, voxels n and n + 1 hold voxel n - 1, as the edge cells do).
+constexpr int kVgEdge = 8;
+constexpr int kVgL = kVgEdge + 4;
+constexpr int kVgCube = kVgL * kVgL * kVgL;
+constexpr int kVgViews = 64;  // view boxes held in LDS at a time
+
+__device__ __forceinline__ void vg_add(unsigned long long* s, float v) {
+  const long long q = (long long)(int)__builtin_rintf(v);
+  __hip_atomic_fetch_add(s, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 template <int L>
 __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
+  __shared__ unsigned long long s_acc[kVgCube];
+  __shared__ int s_box[kVgViews][3];
+  __shared__ int s_pre[kVgViews + 1];
   const VolGeom& g = a.b.g;
-  const long long vid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nvox = (long long)g.n[0] * g.n[1] * g.n[2];
-  if (vid >= nvox) return;
-  const int i = (int)(vid % g.n[0]);
-  const int jj = (int)((vid / g.n[0]) % g.n[1]);
-  const int k = (int)(vid / ((long long)g.n[0] * g.n[1]));
+  const int nb[3] = {(g.n[0] + kVgEdge - 1) / kVgEdge, (g.n[1] + kVgEdge - 1) / kVgEdge,
+                     (g.n[2] + kVgEdge - 1) / kVgEdge};
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int bi[3] = {bid % nb[0], (bid / nb[0]) % nb[1], bid / (nb[0] * nb[1])};
+  const int o[3] = {bi[0] * kVgEdge, bi[1] * kVgEdge, bi[2] * kVgEdge};
+  const int tid = threadIdx.x;
+  const int k = a.scale_log2[0];
   const Cells<float, L, true> vol = cells_of<Cells<float, L, true>>(a.b.vol);
   // With mu = value (slope > 0, intercept 0) a sample is accumulated exactly
-  // when its trilinear value is > 0.  Samples in this voxel's support read
-  // cells inside [i-1, i+1]^3, so if every value there is <= 0 none is.
+  // when its trilinear value is > 0; the samples weighing in this brick
+  // interpolate voxels [8b - 1, 8b + 8], so if all of those are <= 0 none is.
   bool live = true;
   if (a.b.m.intercept == 0.0f && a.b.m.slope > 0.0f) {
     float mx = -1.0f;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int xi = min(max(i + dx, 0), g.n[0] - 1), yi = min(max(jj + dy, 0), g.n[1] - 1);
-          const int zi = min(max(k + dz, 0), g.n[2] - 1);
-          mx = fmaxf(mx, load_voxel(vol, g, xi, yi, zi));
-        }
-    live = mx > 0.0f;
+    for (int e = tid; e < 1000; e += 256) {
+      const int xi = min(max(o[0] - 1 + e % 10, 0), g.n[0] - 1);
+      const int yi = min(max(o[1] - 1 + (e / 10) % 10, 0), g.n[1] - 1);
+      const int zi = min(max(o[2] - 1 + e / 100, 0), g.n[2] - 1);
+      mx = fmaxf(mx, load_voxel(vol, g, xi, yi, zi));
+    }
+    live = __syncthreads_or(mx > 0.0f);
   }
-  float acc = 0.0f;
-  if (live) {
-    // the support in cell coordinates c = p + 0.5: (i, i+2) per axis around
-    // the voxel's centre i + 1, reaching down to the cells below c = 1 for
-    // voxel 0 and up to those above c = n for voxel n - 1 (edge cells hold
-    // the clamped voxel twice)
-    const int vi[3] = {i, jj, k};
+  if (live && k != kVgNonFinite) {
+    for (int e = tid; e < kVgCube; e += 256) s_acc[e] = 0ull;
+    // the cells whose samples weigh in the brick's voxels: (8b, 8b + 9) per
+    // axis, reaching below c = 1 and above c = n at the faces
     float clo[3], chi[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-      clo[ax] = vi[ax] == 0 ? -1.0f : (float)vi[ax];
-      chi[ax] = vi[ax] == g.n[ax] - 1 ? (float)(g.n[ax] + 2) : (float)(vi[ax] + 2);
+      clo[ax] = bi[ax] == 0 ? -1.0f : (float)o[ax];
+      chi[ax] = bi[ax] == nb[ax] - 1 ? (float)(g.n[ax] + 2) : (float)(o[ax] + kVgEdge + 1);
     }
-    for (int v = 0; v < a.b.n_views; ++v) {
-      const BatchView& bv = a.b.views[v];
-      const int W = bv.v.W, H = bv.v.H;
-      // pixel box of the support's 8 corners (in sample coordinates p = c -
-      // 0.5).  Pixel x's ray projects to x exactly, so the box keeps a
-      // margin of a whole pixel beyond the rays that can meet the support,
-      // and v_rcp_f32 (1 ulp) serves for the perspective divide
-      float xmin = 1e30f, xmax = -1e30f, ymin = 1e30f, ymax = -1e30f;
-      bool behind = false;
-      for (int c = 0; c < 8; ++c) {
-        const float px = ((c & 1) ? chi[0] : clo[0]) - 0.5f, py = ((c & 2) ? chi[1] : clo[1]) - 0.5f;
-        const float pz = ((c & 4) ? chi[2] : clo[2]) - 0.5f;
-        const float* M = bv.inv;
-        const float hx = M[0] * px + M[1] * py + M[2] * pz + M[3];
-        const float hy = M[4] * px + M[5] * py + M[6] * pz + M[7];
-        const float hw = M[12] * px + M[13] * py + M[14] * pz + M[15];
-        if (!(hw > 0.0f)) { behind = true; break; }
-        const float rw = __builtin_amdgcn_rcpf(hw);
-        const float vx = hx * rw, vy = hy * rw;
-        const float xp = bv.v.flipped ? vx * (float)W : (1.0f - vx) * (float)W;
-        const float yp = vy * (float)H;
-        xmin = fminf(xmin, xp); xmax = fmaxf(xmax, xp);
-        ymin = fminf(ymin, yp); ymax = fmaxf(ymax, yp);
+    for (int v0 = 0; v0 < a.b.n_views; v0 += kVgViews) {
+      const int nv = min(kVgViews, a.b.n_views - v0);
+      __syncthreads();
+      if (tid < nv) {
+        // pixel box of the region's 8 corners (sample coordinates p = c -
+        // 0.5), a pixel of margin beyond the rays that can meet it
+        const BatchView& bv = a.b.views[v0 + tid];
+        const int W = bv.v.W, H = bv.v.H;
+        float xmin = 1e30f, xmax = -1e30f, ymin = 1e30f, ymax = -1e30f;
+        bool behind = false;
+        for (int c = 0; c < 8; ++c) {
+          const float px = ((c & 1) ? chi[0] : clo[0]) - 0.5f, py = ((c & 2) ? chi[1] : clo[1]) - 0.5f;
+          const float pz = ((c & 4) ? chi[2] : clo[2]) - 0.5f;
+          const float* M = bv.inv;
+          const float hx = M[0] * px + M[1] * py + M[2] * pz + M[3];
+          const float hy = M[4] * px + M[5] * py + M[6] * pz + M[7];
+          const float hw = M[12] * px + M[13] * py + M[14] * pz + M[15];
+          if (!(hw > 0.0f)) { behind = true; break; }
+          const float vx = hx / hw, vy = hy / hw;
+          const float xp = bv.v.flipped ? vx * (float)W : (1.0f - vx) * (float)W;
+          const float yp = vy * (float)H;
+          xmin = fminf(xmin, xp); xmax = fmaxf(xmax, xp);
+          ymin = fminf(ymin, yp); ymax = fmaxf(ymax, yp);
+        }
+        int x0 = 0, x1 = W - 1, y0 = 0, y1 = H - 1;
+        if (!behind) {
+          x0 = max(0, (int)floorf(xmin) - 1); x1 = min(W - 1, (int)ceilf(xmax) + 1);
+          y0 = max(0, (int)floorf(ymin) - 1); y1 = min(H - 1, (int)ceilf(ymax) + 1);
+        }
+        const int w = max(0, x1 - x0 + 1), h = max(0, y1 - y0 + 1);
+        s_box[tid][0] = x0;
+        s_box[tid][1] = y0;
+        s_box[tid][2] = w;
+        s_pre[tid + 1] = w * h;
       }
-      int x0 = 0, x1 = W - 1, y0 = 0, y1 = H - 1;
-      if (!behind) {
-        x0 = max(0, (int)floorf(xmin) - 1); x1 = min(W - 1, (int)ceilf(xmax) + 1);
-        y0 = max(0, (int)floorf(ymin) - 1); y1 = min(H - 1, (int)ceilf(ymax) + 1);
+      __syncthreads();
+      if (tid == 0) {
+        s_pre[0] = 0;
+        for (int v = 0; v < nv; ++v) s_pre[v + 1] += s_pre[v];
       }
-      const float4* rec = reinterpret_cast<const float4*>(a.rays + bv.pix_off);
-      for (int py = y0; py <= y1; ++py)
-        for (int px = x0; px <= x1; ++px) {
-          const long long q = 3 * ((long long)py * W + px);
-          const float4 A = rec[q], B = rec[q + 1], K = rec[q + 2];
-          const int n = __float_as_int(B.w);
-          if (n <= 0 || A.w == 0.0f) continue;
-          // sample indices m with c0 + m inc inside the open support box: the
-          // entry and exit times from v_rcp_f32 reciprocals differ from the
-          // quotients by a few ulp, far less than the one-sample margin each
-          // side, and every sample in the range is tested by its weight
-          float tlo = 0.0f, thi = (float)(n - 1);
-          const float p0[3] = {A.x, A.y, A.z}, in[3] = {B.x, B.y, B.z};
+      __syncthreads();
+      // the boxes' rays, view after view, one per thread
+      const int total = s_pre[nv];
+      int cv = 0;
+      for (int r = tid; r < total; r += 256) {
+        while (r >= s_pre[cv + 1]) ++cv;
+        const int local = r - s_pre[cv];
+        const int bw = s_box[cv][2];
+        const int ly = local / bw;
+        const int px = s_box[cv][0] + (local - ly * bw), py = s_box[cv][1] + ly;
+        const BatchView& bv = a.b.views[v0 + cv];
+        const float4* rec = reinterpret_cast<const float4*>(a.rays + bv.pix_off);
+        const long long q = 3 * ((long long)py * bv.v.W + px);
+        const float4 A = rec[q], B = rec[q + 1], K = rec[q + 2];
+        const int n = __float_as_int(B.w);
+        if (n <= 0 || A.w == 0.0f) continue;
+        // sample indices m with c0 + m inc inside the open region: the entry
+        // and exit times from v_rcp_f32 reciprocals differ from the
+        // quotients by a few ulp, far less than the one-sample margin each
+        // side, and a sample outside adds only to the halo
+        float tlo = 0.0f, thi = (float)(n - 1);
+        const float p0[3] = {A.x, A.y, A.z}, in[3] = {B.x, B.y, B.z};
 #pragma unroll
-          for (int ax = 0; ax < 3; ++ax) {
-            if (in[ax] != 0.0f) {
-              const float rin = __builtin_amdgcn_rcpf(in[ax]);
-              const float ta = (clo[ax] - p0[ax]) * rin, tb = (chi[ax] - p0[ax]) * rin;
-              tlo = fmaxf(tlo, fminf(ta, tb));
-              thi = fminf(thi, fmaxf(ta, tb));
-            } else if (p0[ax] <= clo[ax] || p0[ax] >= chi[ax]) {
-              thi = -1.0f;
-            }
-          }
-          const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(n - 1, (int)ceilf(thi) + 1);
-          for (int m = m0; m <= m1; ++m) {
-            const float t = (float)m;
-            // samples in the keyhole window are not accumulated by the forward
-            if (t >= K.z && t < K.w) continue;
-            const float qx = fmaf(t, B.x, A.x), qy = fmaf(t, B.y, A.y), qz = fmaf(t, B.z, A.z);
-            const float w = axis_weight(qx, i, g.n[0]) * axis_weight(qy, jj, g.n[1]) * axis_weight(qz, k, g.n[2]);
-            if (w <= 0.0f) continue;
-            const float alpha = fmaf(K.x, trilinear(vol, g, qx, qy, qz), K.y);
-            if (alpha < 0.0f) acc += A.w * w;
+        for (int ax = 0; ax < 3; ++ax) {
+          if (in[ax] != 0.0f) {
+            const float rin = __builtin_amdgcn_rcpf(in[ax]);
+            const float ta = (clo[ax] - p0[ax]) * rin, tb = (chi[ax] - p0[ax]) * rin;
+            tlo = fmaxf(tlo, fminf(ta, tb));
+            thi = fminf(thi, fmaxf(ta, tb));
+          } else if (p0[ax] <= clo[ax] || p0[ax] >= chi[ax]) {
+            thi = -1.0f;
           }
         }
+        const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(n - 1, (int)ceilf(thi) + 1);
+        const float coef = ldexpf(A.w, k);
+        for (int m = m0; m <= m1; ++m) {
+          const float t = (float)m;
+          // samples in the keyhole window are not accumulated by the forward
+          if (t >= K.z && t < K.w) continue;
+          const float qx = fmaf(t, B.x, A.x), qy = fmaf(t, B.y, A.y), qz = fmaf(t, B.z, A.z);
+          // the forward's sample value (precise march: no clamp needed)
+          const float alpha = fmaf(K.x, trilinear<false>(vol, g, qx, qy, qz), K.y);
+          if (!(alpha < 0.0f)) continue;
+          // corners ix - 1 (weight 1 - f) and ix (weight f) per axis
+          const float fx = __builtin_amdgcn_fractf(qx), fy = __builtin_amdgcn_fractf(qy);
+          const float fz = __builtin_amdgcn_fractf(qz);
+          const int lx = min(max((int)qx - o[0] + 1, 0), kVgL - 2);
+          const int ly2 = min(max((int)qy - o[1] + 1, 0), kVgL - 2);
+          const int lz = min(max((int)qz - o[2] + 1, 0), kVgL - 2);
+          const float cz0 = coef * (1.0f - fz), cz1 = coef * fz;
+          const float c00 = cz0 * (1.0f - fy), c10 = cz0 * fy, c01 = cz1 * (1.0f - fy), c11 = cz1 * fy;
+          const float wx0 = 1.0f - fx;
+          unsigned long long* s = s_acc + (lz * kVgL + ly2) * kVgL + lx;
+          vg_add(s, c00 * wx0);
+          vg_add(s + 1, c00 * fx);
+          vg_add(s + kVgL, c10 * wx0);
+          vg_add(s + kVgL + 1, c10 * fx);
+          vg_add(s + kVgL * kVgL, c01 * wx0);
+          vg_add(s + kVgL * kVgL + 1, c01 * fx);
+          vg_add(s + kVgL * kVgL + kVgL, c11 * wx0);
+          vg_add(s + kVgL * kVgL + kVgL + 1, c11 * fx);
+        }
+      }
     }
+    // fold the clamped edge voxels at the volume's faces, one axis at a time
+    const int st[3] = {1, kVgL, kVgL * kVgL};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      __syncthreads();
+      const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+      if (tid < kVgL * kVgL) {
+        const int base = (tid % kVgL) * st[a1] + (tid / kVgL) * st[a2];
+        if (bi[ax] == 0) s_acc[base + 2 * st[ax]] += s_acc[base + st[ax]];
+        if (bi[ax] == nb[ax] - 1) {
+          const int nl = g.n[ax] - o[ax] + 1;  // voxel n - 1
+          s_acc[base + nl * st[ax]] += s_acc[base + (nl + 1) * st[ax]] + s_acc[base + (nl + 2) * st[ax]];
+        }
+      }
+    }
+    __syncthreads();
   }
-  a.grad[vid] = acc;
+  for (int e = tid; e < kVgEdge * kVgEdge * kVgEdge; e += 256) {
+    const int lx = e % kVgEdge, ly = (e / kVgEdge) % kVgEdge, lz = e / (kVgEdge * kVgEdge);
+    const int x = o[0] + lx, y = o[1] + ly, z = o[2] + lz;
+    if (x >= g.n[0] || y >= g.n[1] || z >= g.n[2]) continue;
+    float val = 0.0f;
+    if (live) {
+      val = k == kVgNonFinite ? __builtin_nanf("")
+                              : (float)ldexp((double)(long long)s_acc[((lz + 2) * kVgL + ly + 2) * kVgL + lx + 2], -k);
+    }
+    a.grad[((long long)z * g.n[1] + y) * g.n[0] + x] = val;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -658,6 +758,8 @@ struct rvk_drr_plan {
   double* d_stats = nullptr;  // n_views x 8
   float* d_pose_part = nullptr;  // n_items x 6
   RayRec* d_rays = nullptr;      // per pixel, allocated by the first volume gradient
+  float* d_vg_cmax = nullptr;    // per 8x8 tile, the volume gradient's largest |coef|, allocated with d_rays
+  int* d_vg_scale = nullptr;     // its fixed-point scale exponent
   float* d_jac = nullptr;        // 6 x total pixels, allocated with the plan when it has pose derivatives
   int schedule = RVK_SCHED_GRID;
   int waves_per_simd = 4;
@@ -1031,6 +1133,8 @@ int rvk_drr_plan_destroy(rvk_drr_plan* p) {
   hipFree(p->d_stats);
   hipFree(p->d_pose_part);
   if (p->d_rays) hipFree(p->d_rays);
+  if (p->d_vg_cmax) hipFree(p->d_vg_cmax);
+  if (p->d_vg_scale) hipFree(p->d_vg_scale);
   if (p->d_jac) hipFree(p->d_jac);
   if (p->d_pose_geom) hipFree(p->d_pose_geom);
   if (p->d_poses) hipFree(p->d_poses);
@@ -1305,16 +1409,25 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     if (p->march.skip_rule || p->drr.ref_termination)
       return fail(RVK_ERR_UNSUPPORTED, "the volume gradient needs the plain march (no skip rule, no termination)");
     const long long total = p->pix_off[p->n_views];
-    if (!p->d_rays) RVK_HIP(hipMalloc((void**)&p->d_rays, sizeof(RayRec) * (size_t)total));
+    if (!p->d_rays) {
+      RVK_HIP(hipMalloc((void**)&p->d_rays, sizeof(RayRec) * (size_t)total));
+      RVK_HIP(hipMalloc((void**)&p->d_vg_cmax, sizeof(float) * (size_t)p->n_items));
+      RVK_HIP(hipMalloc((void**)&p->d_vg_scale, sizeof(int)));
+    }
     VolGradArgs va;
     va.b = batch_args(p, nullptr);
     va.img = images;
     va.dldi = dldi;
     va.rays = p->d_rays;
+    va.coef_max = p->d_vg_cmax;
+    va.scale_log2 = p->d_vg_scale;
     va.grad = grad_vol;
     hipLaunchKernelGGL(ray_record_kernel, dim3(blocks), dim3(256), 0, s, va);
-    const long long nvox = (long long)p->vol->dims[0] * p->vol->dims[1] * p->vol->dims[2];
-    const dim3 vgrid((unsigned)((nvox + 255) / 256));
+    hipLaunchKernelGGL(vg_scale_kernel, dim3(1), dim3(1024), 0, s, (const float*)p->d_vg_cmax, p->n_items,
+                       p->d_vg_scale);
+    const int* d = p->vol->dims;
+    const dim3 vgrid((unsigned)(((d[0] + kVgEdge - 1) / kVgEdge) * ((d[1] + kVgEdge - 1) / kVgEdge) *
+                                ((d[2] + kVgEdge - 1) / kVgEdge)));
     if (p->vol->layout == kCellLayoutBrick4) hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutBrick4>, vgrid, dim3(256), 0, s, va);
     else if (p->vol->layout == kCellLayoutPair8) hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPair8>, vgrid, dim3(256), 0, s, va);
     else hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPlain>, vgrid, dim3(256), 0, s, va);

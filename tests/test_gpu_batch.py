@@ -385,6 +385,40 @@ def test_volume_gradient_vs_fp64_fd(gpu_ctx):
     plan.close()
 
 
+def test_volume_gradient_repeatable_zero_and_non_finite(gpu_ctx):
+    """The volume gradient sums in fixed point, so two calls give the same
+    bits whatever order the workgroups' waves add in; a zero d loss / d image
+    gives an exactly zero gradient, and a NaN on one ray (with samples) gives
+    NaN, not a silently finite gradient."""
+    torch = _torch()
+    s, tg = _registration_scene(n_views=2, scale=0.1)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    plan = _plan_with_derivatives(gpu_ctx, s, vol)
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    plan.render(out)
+    rng = np.random.default_rng(77)
+    dldi = torch.from_numpy(rng.standard_normal(plan.total).astype(np.float32)).cuda()
+    g1 = torch.full((s.volume.size,), 3.0, dtype=torch.float32, device="cuda")
+    g2 = torch.full_like(g1, -5.0)
+    plan.backward(out, dldi, grad_volume=g1)
+    plan.backward(out, dldi, grad_volume=g2)
+    torch.cuda.synchronize()
+    assert torch.equal(g1, g2)
+    assert bool(torch.isfinite(g1).all()) and int((g1 != 0).sum()) > 0
+    plan.backward(out, torch.zeros_like(dldi), grad_volume=g2)
+    torch.cuda.synchronize()
+    assert int((g2 != 0).sum()) == 0
+    # a pixel at the image centre sees the body
+    bad = dldi.clone()
+    img = s.extra["img"]
+    bad[(img // 2) * img + img // 2] = float("nan")
+    plan.backward(out, bad, grad_volume=g2)
+    torch.cuda.synchronize()
+    assert bool(torch.isnan(g2).any())
+    plan.close()
+    vol.close()
+
+
 def test_register_and_volume_gradient_identical_across_cell_layouts(gpu_ctx):
     torch = _torch()
     s, tg = _registration_scene(n_views=2, scale=0.1)

@@ -419,6 +419,46 @@ def test_volume_gradient_repeatable_zero_and_non_finite(gpu_ctx):
     vol.close()
 
 
+def test_volume_gradient_many_views_additive_and_homogeneous(gpu_ctx):
+    """70 views (more than one LDS chunk of view boxes) on a small positive
+    f32 volume whose edge 8-voxel bricks are partial: the gradient of the
+    70-view plan is the sum of those of its two 35-view halves (to the
+    fixed-point rounding), and doubling d loss / d image doubles the
+    gradient bit for bit (the fixed-point scale moves by one power of two)."""
+    torch = _torch()
+    dims = (21, 19, 18)
+    rng = np.random.default_rng(31)
+    mu = (0.01 + 0.02 * rng.random(dims[::-1])).astype(np.float32)
+    spacing = (1.0, 1.0, 1.0)
+    march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": 0, "precise": 1}
+    drr = {"intercept": 0.0, "slope": 1.0, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L, "ref_termination": 0}
+    c = configs.volume_center(dims, spacing)
+    views = [abi.compute_view(configs.drr_camera(c, 2.0 * np.pi * i / 70, 16, sod=60.0, sid=100.0, fov=40.0,
+                                                 half_extent=20.0), 16, 16, spacing=spacing) for i in range(70)]
+    vol = abi.Volume.from_array(gpu_ctx, mu, spacing)
+
+    # the same per-pixel weights for the whole plan and its halves
+    w_all = np.random.default_rng(5).standard_normal(70 * 256).astype(np.float32)
+
+    def grad_w(vs, w):
+        plan = abi.DrrPlan(gpu_ctx, vol, vs, configs.march_params(march), configs.drr_params(drr))
+        out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+        plan.render(out)
+        g = torch.zeros(mu.size, dtype=torch.float32, device="cuda")
+        plan.backward(out, torch.from_numpy(w).cuda(), grad_volume=g)
+        torch.cuda.synchronize()
+        plan.close()
+        return g.cpu().numpy().astype(np.float64)
+
+    ga = grad_w(views, w_all)
+    gb = grad_w(views[:35], w_all[:35 * 256]) + grad_w(views[35:], w_all[35 * 256:])
+    scale = np.abs(ga).max()
+    assert scale > 0 and np.count_nonzero(ga) > 0.5 * ga.size
+    assert np.abs(ga - gb).max() <= 1e-5 * scale, np.abs(ga - gb).max() / scale
+    assert np.array_equal(grad_w(views, 2.0 * w_all), 2.0 * ga)
+    vol.close()
+
+
 def test_register_and_volume_gradient_identical_across_cell_layouts(gpu_ctx):
     torch = _torch()
     s, tg = _registration_scene(n_views=2, scale=0.1)

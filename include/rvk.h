@@ -377,9 +377,10 @@ int rvk_drr_plan_set_ncc(rvk_drr_plan* plan, int mode);
 /* Adjoint of the batched render: given the rendered images and d loss / d
  * image (packed), write d loss / d pose (n_views x 6 floats, pose as in
  * rvk_pose_matrix; RVK_ERR_NOT_READY when the plan has no dview) and/or
- * d loss / d volume (voxel values, x fastest; float32 volumes with the plain
- * march only; samples in a view's keyhole window are excluded, as in the
- * forward).  For the pose gradient rays are re-marched with the forward's
+ * d loss / d volume (float32 per voxel, x fastest: the derivative with respect
+ * to the stored voxel value, before the plan's intercept and slope, for every
+ * voxel type; plain march only; samples in a view's keyhole window are
+ * excluded, as in the forward).  For the pose gradient rays are re-marched with the forward's
  * control flow; nothing per sample is stored.
  * The volume gradient is brick-driven: each 8^3-voxel brick re-marches the
  * stretch of every ray that meets the cells its voxels weigh in and adds each

@@ -396,8 +396,9 @@ __global__ __launch_bounds__(256) void jac_grad_kernel(JacGradArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Volume adjoint (f32 mu volumes, plain march): per pixel a ray record, then
-// a brick-driven scatter.  d loss / d mu_ijk = sum_views sum_pixels coef_p *
+// Volume adjoint (any voxel type, plain march): per pixel a ray record, then
+// a brick-driven scatter.  d loss / d v_ijk (v the stored voxel value,
+// alpha = cs v + ci) = sum_views sum_pixels coef_p *
 // sum_m [alpha(p_m) < 0] w_ijk(p_m), w the trilinear weight of voxel ijk at
 // the sample and coef_p = d loss / d L_p * ds * slope.  One workgroup owns an
 // 8^3-voxel brick: it marches every ray that can meet the cells its voxels
@@ -504,11 +505,11 @@ __device__ __forceinline__ void vg_add(unsigned long long* s, float v) {
   __hip_atomic_fetch_add(s, (unsigned long long)q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int L>
+template <typename CV>
 __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
   __shared__ unsigned long long s_acc[kVgCube];
   __shared__ int s_box[kVgViews][3];
-  __shared__ int s_pre[kVgViews + 1];
+  __shared__ long long s_pre[kVgViews + 1];  // box ray counts can pass 2^31 (whole images)
   const VolGeom& g = a.b.g;
   const int nb[3] = {(g.n[0] + kVgEdge - 1) / kVgEdge, (g.n[1] + kVgEdge - 1) / kVgEdge,
                      (g.n[2] + kVgEdge - 1) / kVgEdge};
@@ -517,10 +518,12 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
   const int o[3] = {bi[0] * kVgEdge, bi[1] * kVgEdge, bi[2] * kVgEdge};
   const int tid = threadIdx.x;
   const int k = a.scale_log2[0];
-  const Cells<float, L, true> vol = cells_of<Cells<float, L, true>>(a.b.vol);
+  const CV vol = cells_of<CV>(a.b.vol);
   // With mu = value (slope > 0, intercept 0) a sample is accumulated exactly
   // when its trilinear value is > 0; the samples weighing in this brick
   // interpolate voxels [8b - 1, 8b + 8], so if all of those are <= 0 none is.
+  // (A voxel is read as the trilinear value at its own cell corner, weights
+  // exactly 0, which is its value in every cell encoding.)
   bool live = true;
   if (a.b.m.intercept == 0.0f && a.b.m.slope > 0.0f) {
     float mx = -1.0f;
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
       const int xi = min(max(o[0] - 1 + e % 10, 0), g.n[0] - 1);
       const int yi = min(max(o[1] - 1 + (e / 10) % 10, 0), g.n[1] - 1);
       const int zi = min(max(o[2] - 1 + e / 100, 0), g.n[2] - 1);
-      mx = fmaxf(mx, load_voxel(vol, g, xi, yi, zi));
+      mx = fmaxf(mx, trilinear<false>(vol, g, (float)(xi + 1), (float)(yi + 1), (float)(zi + 1)));
     }
     live = __syncthreads_or(mx > 0.0f);
   }
@@ -575,7 +578,7 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
         s_box[tid][0] = x0;
         s_box[tid][1] = y0;
         s_box[tid][2] = w;
-        s_pre[tid + 1] = w * h;
+        s_pre[tid + 1] = (long long)w * h;
       }
       __syncthreads();
       if (tid == 0) {
@@ -584,11 +587,11 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
       }
       __syncthreads();
       // the boxes' rays, view after view, one per thread
-      const int total = s_pre[nv];
+      const long long total = s_pre[nv];
       int cv = 0;
-      for (int r = tid; r < total; r += 256) {
+      for (long long r = tid; r < total; r += 256) {
         while (r >= s_pre[cv + 1]) ++cv;
-        const int local = r - s_pre[cv];
+        const int local = (int)(r - s_pre[cv]);
         const int bw = s_box[cv][2];
         const int ly = local / bw;
         const int px = s_box[cv][0] + (local - ly * bw), py = s_box[cv][1] + ly;
@@ -1404,8 +1407,6 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     RVK_HIP(hipGetLastError());
   }
   if (grad_vol) {
-    if (p->vol->dtype != RVK_F32)
-      return fail(RVK_ERR_UNSUPPORTED, "the volume gradient is defined for float32 volumes");
     if (p->march.skip_rule || p->drr.ref_termination)
       return fail(RVK_ERR_UNSUPPORTED, "the volume gradient needs the plain march (no skip rule, no termination)");
     const long long total = p->pix_off[p->n_views];
@@ -1428,9 +1429,12 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     const int* d = p->vol->dims;
     const dim3 vgrid((unsigned)(((d[0] + kVgEdge - 1) / kVgEdge) * ((d[1] + kVgEdge - 1) / kVgEdge) *
                                 ((d[2] + kVgEdge - 1) / kVgEdge)));
-    if (p->vol->layout == kCellLayoutBrick4) hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutBrick4>, vgrid, dim3(256), 0, s, va);
-    else if (p->vol->layout == kCellLayoutPair8) hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPair8>, vgrid, dim3(256), 0, s, va);
-    else hipLaunchKernelGGL(volume_grad_kernel<kCellLayoutPlain>, vgrid, dim3(256), 0, s, va);
+    int rc = dispatch_frame(p->vol, p->frame, [&](auto tag) {
+      using CV = decltype(tag);
+      hipLaunchKernelGGL(volume_grad_kernel<CV>, vgrid, dim3(256), 0, s, va);
+      return (int)RVK_OK;
+    });
+    if (rc) return rc;
     RVK_HIP(hipGetLastError());
   }
   return RVK_OK;

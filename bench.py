@@ -582,7 +582,7 @@ def main() -> int:
     if not args.no_batched:
         log("timing c4: 64 poses, uneven detectors, batched vs per-pose")
         scenes["c4"] = configs.scene_c4(scale=1.0, with_derivatives=True)
-        c4, imgs = bench_batched(scenes["c4"])
+        c4, imgs = bench_batched(scenes["c4"], volume_gradient=True)
         gpu["c4"] = {"images": imgs}
         log("timing c5: 16-view registration step")
         scenes["c5"] = configs.scene_c5(scale=1.0)

@@ -620,7 +620,15 @@ __global__ __launch_bounds__(256) void volume_grad_kernel(VolGradArgs a) {
         }
         const int m0 = max(0, (int)floorf(tlo) - 1), m1 = min(n - 1, (int)ceilf(thi) + 1);
         const float coef = ldexpf(A.w, k);
-        for (int m = m0; m <= m1; ++m) {
+        // each lane starts its stretch at its own sample (3 x its lane, mod
+        // the length) and wraps round: a wave's rays enter the region
+        // together, and marched in step they would add to the same voxels in
+        // the same instructions (LDS address conflicts); the sum is the same
+        // in any order
+        const int cnt = m1 - m0 + 1;
+        int mm = m0 + (cnt > 0 ? (int)(3 * (threadIdx.x & 63)) % cnt : 0);
+        for (int i = 0; i < cnt; ++i, mm = (mm == m1) ? m0 : mm + 1) {
+          const int m = mm;
           const float t = (float)m;
           // samples in the keyhole window are not accumulated by the forward
           if (t >= K.z && t < K.w) continue;

@@ -641,6 +641,28 @@ def test_pose_backward_needs_dview(gpu_ctx):
     vol.close()
 
 
+def test_volume_backward_rejects_marches_it_does_not_differentiate(gpu_ctx):
+    """The volume gradient is that of the precise plain march: a plan with the
+    skip rule, with the reference's termination or with the reference FP32
+    march is refused with RVK_ERR_UNSUPPORTED (6), before anything runs."""
+    torch = _torch()
+    s = configs.scene_c4(scale=0.0625, n_poses=2)
+    vol = H.gpu_volume(gpu_ctx, s.volume, s.spacing)
+    g = torch.full((s.volume.size,), 4.0, dtype=torch.float32, device="cuda")
+    for march, drr in ((dict(s.march, skip_rule=1), s.drr), (dict(s.march, precise=0), s.drr),
+                       (s.march, dict(s.drr, ref_termination=1))):
+        plan = abi.DrrPlan(gpu_ctx, vol, s.views, configs.march_params(march), configs.drr_params(drr))
+        out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+        plan.render(out)
+        with pytest.raises(abi.RvkError) as e:
+            plan.backward(out, torch.ones_like(out), grad_volume=g)
+        assert e.value.status == 6
+        plan.close()
+    torch.cuda.synchronize()
+    assert bool((g == 4.0).all())
+    vol.close()
+
+
 def test_pose_gradient_at_h_1e3_on_a_smooth_phantom(gpu_ctx):
     """Against FP64 central differences at h = 1e-3 rad / mm -- the step the
     error budget names -- the pose gradient agrees within 1e-3 for rotations

@@ -78,6 +78,46 @@ def test_drr_all_voxel_types(gpu_ctx, dtype):
     _drr_check(gpu_ctx, vol, s.spacing, s.views[0], s.march, drr)
 
 
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 2, 5), (4, 3, 2), (5, 4, 3), (7, 6, 6)])
+def test_drr_smallest_volumes_ragged_views(gpu_ctx, dims):
+    """The smallest volumes the library accepts (2 or 3 voxels on an axis:
+    the reference's crop box [1, dim - 2] is then empty or flat, so every
+    ray is rejected) and a 7x6x6 volume whose rays do cross it, f32 and
+    int16, on 9x7 views (no tile is whole), skip rule on and off, precise
+    march and the reference's: single-view renders against the oracle, and
+    a two-view batched plan bit for bit against them."""
+    import torch
+    rng = np.random.default_rng(sum(dims))
+    spacing = (1.0, 1.2, 0.9)
+    c = configs.volume_center(dims, spacing)
+    views = [abi.compute_view(configs.drr_camera(c, a, 9, sod=60.0, sid=100.0, fov=10.0, half_extent=8.0), 9, 7,
+                              spacing=spacing) for a in (0.3, 1.9)]
+    for vol, drr in ((rng.uniform(0.0, 0.05, dims[::-1]).astype(np.float32),
+                      {"intercept": 0.0, "slope": 1.0, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L,
+                       "ref_termination": 0}),
+                     (rng.integers(-1000, 1500, dims[::-1]).astype(np.int16),
+                      {"intercept": 0.02, "slope": 2e-5, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L,
+                       "ref_termination": 0})):
+        for skip in (0, 1):
+            for precise in (0, 1):
+                march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": skip, "precise": precise}
+                singles = [_drr_check(gpu_ctx, vol, spacing, v, march, drr) for v in views]
+                if min(dims) <= 3:
+                    assert all(np.all(s == 1.0) for s in singles)
+                else:
+                    assert any(np.any(s < 1.0) for s in singles)
+                vh = H.gpu_volume(gpu_ctx, vol, spacing)
+                plan = abi.DrrPlan(gpu_ctx, vh, views, configs.march_params(march), configs.drr_params(drr))
+                out = torch.full((plan.total,), float("nan"), dtype=torch.float32, device="cuda")
+                plan.render(out)
+                torch.cuda.synchronize()
+                got = out.cpu().numpy()
+                for i, s1 in enumerate(singles):
+                    assert np.array_equal(got[plan.offsets[i]:plan.offsets[i + 1]], s1.ravel()), (skip, precise, i)
+                plan.close()
+                vh.close()
+
+
 @pytest.mark.parametrize("dtype", ["u16", "i16"])
 def test_drr_wide_range_16bit_frames(gpu_ctx, dtype):
     """16-bit frames whose values span 32768 or more keep plain (a, b) cells

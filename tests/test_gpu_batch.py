@@ -627,6 +627,59 @@ def test_volume_gradient_per_voxel_edges_and_keyhole(gpu_ctx):
     vol.close()
 
 
+def test_volume_gradient_per_voxel_single_brick(gpu_ctx):
+    """A 7x6x6 volume is one 8^3 brick of the volume adjoint, so the clamped
+    edge voxels of both faces of every axis fold into that one brick:
+    d loss / d volume of a linear loss, voxel by voxel, against FP64 central
+    differences (the FP64 march replays the FP32 step counts), for two
+    cone-beam views and parallel views along y through the cells next to
+    the low-x and the high-x faces."""
+    torch = _torch()
+    dims = (7, 6, 6)
+    rng = np.random.default_rng(29)
+    mu = (0.02 + 0.03 * rng.random(dims[::-1])).astype(np.float32)
+    spacing = (1.0, 1.2, 0.9)
+    march = {"step_scale": 0.5, "term_T": 0.0, "jitter": 1, "skip_rule": 0, "precise": 1}
+    drr = {"intercept": 0.0, "slope": 1.0, "offset": 1.0, "output": abi.DRR_F32_EXP_NEG_L, "ref_termination": 0}
+    c = configs.volume_center(dims, spacing)
+    views = [abi.compute_view(configs.drr_camera(c, a, 9, sod=60.0, sid=100.0, fov=10.0, half_extent=8.0), 9, 7,
+                              spacing=spacing) for a in (0.3, 1.9)]
+    views.append(_ortho_view(6, 4, 0.1, 4.6, 1.2, 2.6, -2.0, dims[1] + 4.0))
+    views.append(_ortho_view(6, 4, 5.6, 0.9, 1.2, 2.6, -2.0, dims[1] + 4.0))
+    vol = abi.Volume.from_array(gpu_ctx, mu, spacing)
+    plan = abi.DrrPlan(gpu_ctx, vol, views, configs.march_params(march), configs.drr_params(drr))
+    out = torch.empty(plan.total, dtype=torch.float32, device="cuda")
+    plan.render(out)
+    w = torch.from_numpy(rng.standard_normal(plan.total).astype(np.float32)).cuda()
+    gvol = torch.zeros(mu.size, dtype=torch.float32, device="cuda")
+    plan.backward(out, w, grad_volume=gvol)
+    torch.cuda.synchronize()
+    g = gvol.cpu().numpy().reshape(mu.shape).astype(np.float64)
+    wv = w.cpu().numpy().astype(np.float64)
+    vds = [configs.view_dict(v) for v in views]
+    m64 = dict(march, f32_control=1)
+
+    def loss(vol):
+        return sum(float(orc.drr(vol, spacing, vds[v], m64, drr, "f64", nthreads=1).ravel() @
+                         wv[plan.offsets[v]:plan.offsets[v + 1]]) for v in range(len(views)))
+
+    h = 1e-3
+    fd = np.zeros_like(g)
+    for idx in np.ndindex(*mu.shape):
+        up, dn = mu.copy(), mu.copy()
+        up[idx] += h
+        dn[idx] -= h
+        fd[idx] = (loss(up) - loss(dn)) / (2 * h)
+    scale = np.abs(fd).max()
+    print(f"single-brick volume gradient: max |g - fd| {np.abs(g - fd).max():.3e} of max |fd| {scale:.3e}")
+    assert scale > 0 and np.count_nonzero(fd) > 0.5 * fd.size
+    # both x faces carry gradient
+    assert np.abs(fd[:, :, 0]).max() > 0.05 * scale and np.abs(fd[:, :, -1]).max() > 0.05 * scale
+    np.testing.assert_allclose(g, fd, rtol=1e-3, atol=1e-4 * scale)
+    plan.close()
+    vol.close()
+
+
 def test_pose_backward_needs_dview(gpu_ctx):
     torch = _torch()
     s = configs.scene_c4(scale=0.125, n_poses=2)

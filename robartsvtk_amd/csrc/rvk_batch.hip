@@ -1418,11 +1418,18 @@ int rvk_drr_plan_backward(rvk_drr_plan* p, const float* images, const float* dld
     if (p->march.skip_rule || p->drr.ref_termination)
       return fail(RVK_ERR_UNSUPPORTED, "the volume gradient needs the plain march (no skip rule, no termination)");
     const long long total = p->pix_off[p->n_views];
-    if (!p->d_rays) {
-      RVK_HIP(hipMalloc((void**)&p->d_rays, sizeof(RayRec) * (size_t)total));
-      RVK_HIP(hipMalloc((void**)&p->d_vg_cmax, sizeof(float) * (size_t)p->n_items));
-      RVK_HIP(hipMalloc((void**)&p->d_vg_scale, sizeof(int)));
-    }
+    // each buffer on its own, set only once allocated: after a failed
+    // allocation the next call allocates what is still missing
+    auto need = [](void* have, size_t bytes, void** out) -> hipError_t {
+      if (have) return hipSuccess;
+      void* q = nullptr;
+      const hipError_t e = hipMalloc(&q, bytes);
+      if (e == hipSuccess) *out = q;
+      return e;
+    };
+    RVK_HIP(need(p->d_rays, sizeof(RayRec) * (size_t)total, (void**)&p->d_rays));
+    RVK_HIP(need(p->d_vg_cmax, sizeof(float) * (size_t)p->n_items, (void**)&p->d_vg_cmax));
+    RVK_HIP(need(p->d_vg_scale, sizeof(int), (void**)&p->d_vg_scale));
     VolGradArgs va;
     va.b = batch_args(p, nullptr);
     va.img = images;

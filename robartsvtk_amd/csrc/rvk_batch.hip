@@ -1502,11 +1502,24 @@ int rvk_drr_plan_set_poses(rvk_drr_plan* p, const rvk_camera* cameras, const dou
     p->d_poses = nullptr;
     p->d_opt_state = nullptr;
     p->n_slots = 0;
-    RVK_HIP(hipMalloc((void**)&p->d_poses, sizeof(double) * 6 * n_slots));
-    RVK_HIP(hipMalloc((void**)&p->d_opt_state, sizeof(double) * (12 * (size_t)n_slots + 1)));
+    // both or neither: the plan never holds poses without their optimiser state
+    void* dp = nullptr;
+    void* ds = nullptr;
+    hipError_t e = hipMalloc(&dp, sizeof(double) * 6 * n_slots);
+    if (e == hipSuccess) e = hipMalloc(&ds, sizeof(double) * (12 * (size_t)n_slots + 1));
+    if (e != hipSuccess) {
+      if (dp) hipFree(dp);
+      return hip_fail(e, "hipMalloc(device poses)");
+    }
+    p->d_poses = (double*)dp;
+    p->d_opt_state = (double*)ds;
     p->n_slots = n_slots;
   }
-  if (!p->d_pose_geom) RVK_HIP(hipMalloc((void**)&p->d_pose_geom, sizeof(PoseGeom) * p->n_views));
+  if (!p->d_pose_geom) {
+    void* q = nullptr;
+    RVK_HIP(hipMalloc(&q, sizeof(PoseGeom) * p->n_views));
+    p->d_pose_geom = (PoseGeom*)q;
+  }
   drop_graph(p);
   p->opt = *opt;
   const std::vector<double> state(12 * (size_t)n_slots + 1, 0.0);

@@ -290,7 +290,8 @@ def bench_batched(scene, reps: int = 5, volume_gradient: bool = False, loop: boo
 
 
 # ---------------------------------------------------------------------------
-# CPU baseline and accuracy (the oracle is the checker and the CPU port)
+# CPU baseline and accuracy (the oracle is what the GPU results are tested
+# against, and the CPU port)
 def _median_time(fn, runs: int = 5) -> float:
     fn()
     ts = []

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/librvk_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product package never
+cpu_baseline leg of bench.py, to test against.  The product package never
 imports it.
 
 Views are plain dicts so that the oracle stays independent of the product's

@@ -1,8 +1,8 @@
 /*
  * rvk_oracle.cpp -- CPU restatement of RobartsVTK's CUDA ray casters.
  *
- * TEST INFRASTRUCTURE ONLY (the checker of the HIP casters and the reported
- * CPU baseline).  Nothing in the product library links or loads this file.
+ * TEST INFRASTRUCTURE ONLY (what the HIP casters are tested against, and the
+ * reported CPU baseline).  Nothing in the product library links or loads this file.
  *
  * The reference kernels cannot run here (they need nvcc, legacy CUDA texture
  * references and VTK 7), so this is a transcription written from the source

@@ -1,8 +1,8 @@
 /*
  * rvk_oracle.h -- CPU restatement of RobartsVTK's CUDA ray-casting kernels.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * casters: tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * TEST INFRASTRUCTURE ONLY.  The HIP casters are tested against this
+ * library: tests/, __graft_entry__.smoke() and the cpu_baseline leg of
  * bench.py may load it; the product library (robartsvtk_amd/csrc) never
  * does.  Every function is templated on the working precision and exported
  * in an FP32 and an FP64 flavour.

@@ -485,7 +485,9 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 50 untimed steps (~24 ms of c2) bring the clocks to their steady state;
+    # after 5 the timed steps still run ~2 % slower
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the c4/c5 batching measurements")
     ap.add_argument("--no-casters", action="store_true", help="skip the c1/c3 compositing-caster measurements")
